@@ -1,0 +1,379 @@
+// gqf_api.hip -- host side of libgqf: lifecycle, workspace and the C ABI.
+//
+// The handle returned by qf_malloc_device is a device pointer to a
+// `quotient_filter` descriptor (as the reference's QF* is,
+// src/gqf.cu:2212-2217).  The host keeps a registry from that pointer to
+// the filter's host-side state: sizes, the workspace of the bulk pipeline
+// and the device status word.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_map>
+
+#include "../../include/gqf.h"
+#include "gqf_internal.h"
+#include "gqf_launch.h"
+
+namespace {
+
+constexpr int kModeInsert = 0;
+constexpr int kModeDelete = 1;
+constexpr size_t kBlockPad = 4096;  // vector over-reads past the last block stay inside the allocation
+
+std::mutex g_mu;
+int g_last_error = 0;
+
+void note_error(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_last_error) g_last_error = (int)e;
+  fprintf(stderr, "libgqf: %s failed: %s\n", what, hipGetErrorString(e));
+}
+
+#define GQF_CHECK(call) note_error((call), #call)
+
+struct Workspace {
+  uint64_t cap = 0;            // keys the buffers below are sized for
+  uint64_t* tmp = nullptr;     // sort ping-pong, cap keys
+  uint32_t* hist = nullptr;    // [digit][tile]
+  uint32_t* totals = nullptr;  // 1024 digit totals
+  uint32_t* tile_heads = nullptr;
+  uint64_t* ukeys = nullptr;   // run-length reduce outputs, cap keys each
+  uint64_t* upos = nullptr;
+  uint64_t* ucounts = nullptr;
+  uint64_t* m_dev = nullptr;
+  uint32_t* region_start = nullptr;  // nregions + 1
+  uint8_t* wide_flags = nullptr;     // nregions, kept zero by the kernels
+  uint32_t* wide_count = nullptr;
+  unsigned long long* misses = nullptr;
+};
+
+struct Filter {
+  QF host;           // host mirror of the device descriptor
+  QF* dev = nullptr; // the public handle
+  Workspace ws;
+  std::mutex mu;     // serialises workspace growth
+};
+
+std::unordered_map<QF*, Filter*> g_filters;
+
+Filter* lookup(QF* qf) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_filters.find(qf);
+  if (it == g_filters.end()) {
+    fprintf(stderr, "libgqf: unknown QF handle %p\n", (void*)qf);
+    return nullptr;
+  }
+  return it->second;
+}
+
+template <typename T>
+void dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  GQF_CHECK(hipMalloc((void**)p, count * sizeof(T)));
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) GQF_CHECK(hipFree((void*)p));
+  p = nullptr;
+}
+
+void free_workspace(Workspace& w) {
+  dev_free(w.tmp);
+  dev_free(w.hist);
+  dev_free(w.tile_heads);
+  dev_free(w.ukeys);
+  dev_free(w.upos);
+  dev_free(w.ucounts);
+  w.cap = 0;
+}
+
+// grow the per-key buffers to hold n keys (synchronises when it allocates)
+void ensure_workspace(Filter* f, uint64_t n) {
+  std::lock_guard<std::mutex> lk(f->mu);
+  Workspace& w = f->ws;
+  if (!w.totals) {
+    dev_alloc(&w.totals, 1024);
+    dev_alloc(&w.m_dev, 1);
+    dev_alloc(&w.region_start, f->host.nregions + 2);
+    dev_alloc(&w.wide_flags, f->host.nregions + 1);
+    GQF_CHECK(hipMemset(w.wide_flags, 0, f->host.nregions + 1));
+    dev_alloc(&w.wide_count, 1);
+    GQF_CHECK(hipMemset(w.wide_count, 0, sizeof(uint32_t)));
+    dev_alloc(&w.misses, 1);
+  }
+  if (n <= w.cap) return;
+  GQF_CHECK(hipDeviceSynchronize());
+  free_workspace(w);
+  uint64_t cap = n < 4096 ? 4096 : n;
+  dev_alloc(&w.tmp, cap);
+  dev_alloc(&w.hist, gqf_sort_hist_words(cap));
+  dev_alloc(&w.tile_heads, (cap + 4095) / 4096 + 1);
+  dev_alloc(&w.ukeys, cap);
+  dev_alloc(&w.upos, cap);
+  dev_alloc(&w.ucounts, cap);
+  w.cap = cap;
+}
+
+void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool reduce, hipStream_t s) {
+  if (n == 0) return;  // the reference's grid size underflows here
+  Filter* f = lookup(qf);
+  if (!f) return;
+  ensure_workspace(f, n);
+  Workspace& w = f->ws;
+  const bool hash_it = !(flags & QF_KEY_IS_HASH);
+  const int bits = hash_it ? (int)f->host.key_bits : 64;
+  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : 0, f->host.range - 1, bits, w.hist, w.totals, s);
+  const uint64_t* items = keys;
+  const uint64_t* counts = nullptr;
+  const uint64_t* n_dev = nullptr;
+  if (reduce) {
+    gqf_launch_rle(keys, n, w.tile_heads, w.ukeys, w.upos, w.ucounts, w.m_dev, s);
+    items = w.ukeys;
+    counts = w.ucounts;
+    n_dev = w.m_dev;
+  }
+  gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, s);
+  gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, w.region_start, mode, w.wide_flags, w.wide_count, s);
+  GQF_CHECK(hipGetLastError());
+}
+
+uint64_t count_misses(QF* qf, uint64_t* vals, uint64_t n) {
+  if (n == 0) return 0;
+  Filter* f = lookup(qf);
+  if (!f) return 0;
+  ensure_workspace(f, 0);
+  GQF_CHECK(hipMemsetAsync(f->ws.misses, 0, sizeof(unsigned long long), 0));
+  gqf_launch_misses(f->dev, vals, n, f->ws.misses, 0);
+  unsigned long long h = 0;
+  GQF_CHECK(hipMemcpy(&h, f->ws.misses, sizeof(h), hipMemcpyDeviceToHost));
+  return (uint64_t)h;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// lifecycle
+
+extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
+  *qf = nullptr;
+  if (nbits < 6 || nbits > 40) {
+    fprintf(stderr, "libgqf: nbits %d out of range\n", nbits);
+    return;
+  }
+  Filter* f = new Filter();
+  QF& h = f->host;
+  memset(&h, 0, sizeof(h));
+  // sizing arithmetic of qf_init / qf_malloc_device (src/gqf.cu:1939-1996,
+  // 2128-2140): r = 8 remainder bits, 0 value bits, double-precision sqrt
+  h.nslots = 1ULL << nbits;
+  h.xnslots = h.nslots + (uint64_t)(10 * sqrt((double)h.nslots));
+  h.nblocks = (h.xnslots + GQF_SLOTS_PER_BLOCK - 1) / GQF_SLOTS_PER_BLOCK;
+  h.key_bits = (uint64_t)nbits + 8;
+  h.key_remainder_bits = 8;
+  h.bits_per_slot = 8;
+  h.range = h.nslots << h.key_remainder_bits;
+  h.num_locks = h.xnslots / GQF_REF_LOCK_SLOTS + 2;
+  h.nregions = (h.nblocks + GQF_REGION_BLOCKS - 1) / GQF_REGION_BLOCKS;
+  const size_t bytes = (size_t)h.nblocks * GQF_BLOCK_BYTES;
+  dev_alloc(&h.blocks, bytes + kBlockPad);
+  GQF_CHECK(hipMemset(h.blocks, 0, bytes + kBlockPad));  // qf_malloc memsets the image (src/gqf.cu:2081)
+  dev_alloc(&h.status, 1);
+  GQF_CHECK(hipMemset(h.status, 0, sizeof(uint32_t)));
+  if (!bulk_config) {
+    const size_t nl = (size_t)h.num_locks * GQF_REF_LOCK_DIST;
+    dev_alloc(&h.locks, nl);
+    GQF_CHECK(hipMemset(h.locks, 0, nl * sizeof(uint32_t)));
+  }
+  dev_alloc(&f->dev, 1);
+  GQF_CHECK(hipMemcpy(f->dev, &h, sizeof(QF), hipMemcpyHostToDevice));
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_filters[f->dev] = f;
+  }
+  *qf = f->dev;
+}
+
+extern "C" void qf_destroy_device(QF* qf) {
+  Filter* f = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_filters.find(qf);
+    if (it == g_filters.end()) return;
+    f = it->second;
+    g_filters.erase(it);
+  }
+  GQF_CHECK(hipDeviceSynchronize());
+  free_workspace(f->ws);
+  dev_free(f->ws.totals);
+  dev_free(f->ws.m_dev);
+  dev_free(f->ws.region_start);
+  dev_free(f->ws.wide_flags);
+  dev_free(f->ws.wide_count);
+  dev_free(f->ws.misses);
+  dev_free(f->host.blocks);
+  dev_free(f->host.status);
+  dev_free(f->host.locks);
+  dev_free(f->dev);
+  delete f;
+}
+
+// ---------------------------------------------------------------------------
+// bulk API
+
+extern "C" void bulk_insert(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags) {
+  mutate(qf, nvals, keys, flags, kModeInsert, false, 0);
+}
+
+extern "C" void bulk_insert_reduce(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags) {
+  mutate(qf, nvals, keys, flags, kModeInsert, true, 0);
+}
+
+extern "C" void bulk_delete(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags) {
+  mutate(qf, nvals, keys, flags, kModeDelete, false, 0);
+}
+
+extern "C" void bulk_get(QF* qf, uint64_t nvals, uint64_t* vals, uint64_t* returns) {
+  bulk_get_async(qf, nvals, vals, returns, nullptr);
+}
+
+extern "C" uint64_t bulk_get_misses_wrapper(QF* qf, uint64_t* vals, uint64_t nvals) {
+  return count_misses(qf, vals, nvals);
+}
+
+extern "C" uint64_t point_get_wrapper(QF* qf, uint64_t* hashes, uint64_t nitems) {
+  // point_query(key % range) == 0 for each key (src/gqf.cu:3112-3150); the
+  // invertible hash sees only key & (range - 1), so this is the miss count.
+  return count_misses(qf, hashes, nitems);
+}
+
+// The reference kernel runs one locked point_insert per key
+// (src/gqf.cu:3153-3162).  The image a set of inserts leaves is the same in
+// any order, so the wrapper applies the keys as one bulk insertion of a copy
+// (the caller's array is left as it was, as the reference leaves it).
+extern "C" void point_bulk_insert_wrapper(QF* qf, uint64_t* hashes, uint64_t nitems) {
+  if (nitems == 0) return;
+  Filter* f = lookup(qf);
+  if (!f) return;
+  ensure_workspace(f, nitems);
+  GQF_CHECK(hipMemcpyAsync(f->ws.upos, hashes, nitems * sizeof(uint64_t), hipMemcpyDeviceToDevice, 0));
+  mutate(qf, nitems, f->ws.upos, 0, kModeInsert, false, 0);
+  GQF_CHECK(hipDeviceSynchronize());
+}
+
+extern "C" void bulk_insert_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, void* stream) {
+  mutate(qf, nvals, keys, flags, kModeInsert, false, (hipStream_t)stream);
+}
+
+extern "C" void bulk_insert_reduce_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, void* stream) {
+  mutate(qf, nvals, keys, flags, kModeInsert, true, (hipStream_t)stream);
+}
+
+extern "C" void bulk_delete_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, void* stream) {
+  mutate(qf, nvals, keys, flags, kModeDelete, false, (hipStream_t)stream);
+}
+
+extern "C" void bulk_get_async(QF* qf, uint64_t nvals, uint64_t* vals, uint64_t* returns, void* stream) {
+  if (nvals == 0) return;
+  Filter* f = lookup(qf);
+  if (!f) return;
+  gqf_launch_bulk_get(f->dev, vals, nvals, returns, (hipStream_t)stream);
+  GQF_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// additions
+
+extern "C" int qf_get_info(QF* qf, gqf_info* out) {
+  Filter* f = lookup(qf);
+  if (!f || !out) return -1;
+  const QF& h = f->host;
+  out->nslots = h.nslots;
+  out->xnslots = h.xnslots;
+  out->nblocks = h.nblocks;
+  out->key_bits = h.key_bits;
+  out->key_remainder_bits = h.key_remainder_bits;
+  out->bits_per_slot = h.bits_per_slot;
+  out->range = h.range;
+  out->num_locks = h.num_locks;
+  out->nregions = h.nregions;
+  out->total_size_in_bytes = h.nblocks * GQF_BLOCK_BYTES;
+  return 0;
+}
+
+extern "C" uint32_t qf_get_status(QF* qf) {
+  Filter* f = lookup(qf);
+  if (!f) return 0;
+  uint32_t s = 0;
+  GQF_CHECK(hipDeviceSynchronize());
+  GQF_CHECK(hipMemcpy(&s, f->host.status, sizeof(s), hipMemcpyDeviceToHost));
+  return s;
+}
+
+extern "C" void qf_clear_status(QF* qf) {
+  Filter* f = lookup(qf);
+  if (!f) return;
+  GQF_CHECK(hipDeviceSynchronize());
+  GQF_CHECK(hipMemset(f->host.status, 0, sizeof(uint32_t)));
+}
+
+extern "C" int qf_export_image(QF* qf, void* host_dst, uint64_t nbytes) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  if (nbytes < sz) return -2;
+  GQF_CHECK(hipDeviceSynchronize());
+  hipError_t e = hipMemcpy(host_dst, f->host.blocks, sz, hipMemcpyDeviceToHost);
+  note_error(e, "qf_export_image");
+  return e == hipSuccess ? 0 : -3;
+}
+
+extern "C" int qf_import_image(QF* qf, const void* host_src, uint64_t nbytes) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  if (nbytes != sz) return -2;
+  GQF_CHECK(hipDeviceSynchronize());
+  hipError_t e = hipMemcpy(f->host.blocks, host_src, sz, hipMemcpyHostToDevice);
+  note_error(e, "qf_import_image");
+  return e == hipSuccess ? 0 : -3;
+}
+
+extern "C" int qf_export_image_device(QF* qf, void* dev_dst, uint64_t nbytes, void* stream) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  if (nbytes < sz) return -2;
+  hipError_t e = hipMemcpyAsync(dev_dst, f->host.blocks, sz, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  note_error(e, "qf_export_image_device");
+  return e == hipSuccess ? 0 : -3;
+}
+
+extern "C" int qf_import_image_device(QF* qf, const void* dev_src, uint64_t nbytes, void* stream) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  if (nbytes != sz) return -2;
+  hipError_t e = hipMemcpyAsync(f->host.blocks, dev_src, sz, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  note_error(e, "qf_import_image_device");
+  return e == hipSuccess ? 0 : -3;
+}
+
+extern "C" int qf_reserve(QF* qf, uint64_t max_batch) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  ensure_workspace(f, max_batch);
+  return f->ws.cap >= max_batch ? 0 : -2;
+}
+
+extern "C" int qf_last_error(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int e = g_last_error;
+  g_last_error = 0;
+  return e;
+}

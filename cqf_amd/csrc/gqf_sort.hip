@@ -1,0 +1,321 @@
+// gqf_sort.hip -- hash + LSD radix sort + run-length reduce for the bulk path.
+//
+// Replaces hash_all + thrust::sort + thrust::fill/reduce_by_key of the
+// reference bulk API (src/gqf.cu:2615-2637, 3206-3209, 3258-3284).
+//
+// Sort: least-significant-digit radix sort over only the bits a hash can
+// have (q + 8 of them; all 64 for pre-hashed keys), always an even number of
+// passes so the result lands back in the caller's buffer, as the reference
+// leaves `keys` holding the ascending hashes (src/gqf.cu:3180-3181).
+// Each pass is upsweep (per-tile digit histogram in LDS) -> scan of the
+// [digit][tile] matrix -> downsweep (stable rank with wave ballots, staged in
+// LDS so each digit's keys leave the tile as one contiguous run).
+// The first pass hashes on the fly.
+#include "gqf_internal.h"
+#include "gqf_launch.h"
+
+namespace {
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;  // 4096 keys
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxRadix = 1024;           // digits of up to 10 bits
+
+__device__ inline uint64_t load_key(const uint64_t* in, uint64_t i, uint64_t n, int hash_it, uint64_t mask) {
+  if (i >= n) return ~0ULL;
+  uint64_t k = in[i];
+  return hash_it ? gqf_hash64(k, mask) : k;
+}
+}  // namespace
+
+// per-tile digit histogram -> hist[d * ntiles + tile]
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits, int hash_it,
+                 uint64_t mask, uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t h[kMaxRadix];
+  const int radix = 1 << bits;
+  for (int d = threadIdx.x; d < radix; d += kThreads) h[d] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll 4
+  for (int k = 0; k < kItems; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+    if (i < n) {
+      uint64_t key = load_key(in, i, n, hash_it, mask);
+      atomicAdd(&h[(uint32_t)(key >> shift) & (radix - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < radix; d += kThreads) hist[(uint64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+// Exclusive scan of each digit's row of tile counts; row totals -> totals[d].
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_scan_rows(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[kWaves];
+  uint32_t* row = hist + (uint64_t)blockIdx.x * ntiles;
+  uint32_t carry = 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < ntiles; base += kThreads) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < ntiles ? row[i] : 0;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < kWaves; k++) {
+      if (k < w) before += wsum[k];
+      total += wsum[k];
+    }
+    if (i < ntiles) row[i] = carry + before + incl - v;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Exclusive scan of the digit totals (one workgroup, radix <= 1024).
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_scan_totals(uint32_t* __restrict__ totals, int radix) {
+  __shared__ uint32_t s[kMaxRadix];
+  for (int d = threadIdx.x; d < radix; d += kThreads) s[d] = totals[d];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int d = 0; d < radix; d++) { uint32_t t = s[d]; s[d] = run; run += t; }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < radix; d += kThreads) totals[d] = s[d];
+}
+
+// Stable scatter of one tile.
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int shift,
+                   int bits, int hash_it, uint64_t mask, const uint32_t* __restrict__ hist,
+                   const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+  __shared__ uint32_t wcnt[kWaves][kMaxRadix];
+  __shared__ uint32_t tile_start[kMaxRadix];
+  __shared__ uint32_t glob[kMaxRadix];
+  __shared__ uint64_t stage[kTile];
+  const int radix = 1 << bits;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int d = threadIdx.x; d < radix; d += kThreads) {
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) wcnt[k][d] = 0;
+    glob[d] = bin_base[d] + hist[(uint64_t)d * ntiles + blockIdx.x];
+  }
+  __syncthreads();
+
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  const uint64_t lt_mask = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  uint64_t keys[kItems];
+  uint32_t rank[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+    bool valid = i < n;
+    uint64_t key = load_key(in, i, n, hash_it, mask);
+    uint32_t d = (uint32_t)(key >> shift) & (radix - 1);
+    // lanes holding the same digit (and a valid key)
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; b++) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    uint32_t before = wcnt[w][d];
+    rank[k] = before + (uint32_t)__popcll(peers & lt_mask);
+    keys[k] = key;
+    // lowest peer bumps the wave's digit counter
+    if (valid && (peers & lt_mask) == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // prefix over waves per digit, tile totals -> tile_start (exclusive over digits)
+  for (int d = threadIdx.x; d < radix; d += kThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) {
+      uint32_t t = wcnt[k][d];
+      wcnt[k][d] = run;
+      run += t;
+    }
+    tile_start[d] = run;  // tile count, scanned below
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // exclusive scan of tile_start over `radix` digits by one wave
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < radix; b0 += 64) {
+      uint32_t v = tile_start[b0 + lane];
+      uint32_t incl = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+      }
+      tile_start[b0 + lane] = carry + incl - v;
+      carry += __shfl(incl, 63, 64);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+    if (i < n) {
+      uint32_t d = (uint32_t)(keys[k] >> shift) & (radix - 1);
+      stage[tile_start[d] + wcnt[w][d] + rank[k]] = keys[k];
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)min((uint64_t)kTile, n - base);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
+    uint64_t key = stage[i];
+    uint32_t d = (uint32_t)(key >> shift) & (radix - 1);
+    out[(uint64_t)glob[d] + (i - tile_start[d])] = key;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// run-length reduce of sorted keys: unique keys + their multiplicities
+
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_rle_count(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ tile_heads) {
+  __shared__ uint32_t c;
+  if (threadIdx.x == 0) c = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t mine = 0;
+  for (int k = 0; k < kItems; k++) {
+    uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
+    if (i < n && (i == 0 || keys[i] != keys[i - 1])) mine++;
+  }
+  atomicAdd(&c, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_heads[blockIdx.x] = c;
+}
+
+// exclusive scan of tile head counts (one workgroup); total -> *m
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_rle_scan(uint32_t* __restrict__ tile_heads, uint32_t ntiles, uint64_t* __restrict__ m) {
+  __shared__ uint32_t wsum[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < ntiles; base += kThreads) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < ntiles ? tile_heads[i] : 0;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < kWaves; k++) {
+      if (k < w) before += wsum[k];
+      total += wsum[k];
+    }
+    if (i < ntiles) tile_heads[i] = carry + before + incl - v;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *m = carry;
+}
+
+// write unique keys and the index where each run starts
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_rle_emit(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ tile_base,
+             uint64_t* __restrict__ ukeys, uint64_t* __restrict__ upos) {
+  __shared__ uint32_t wsum[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t carry = tile_base[blockIdx.x];
+  const uint64_t lt_mask = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  for (int k = 0; k < kItems; k++) {
+    uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
+    bool head = i < n && (i == 0 || keys[i] != keys[i - 1]);
+    uint64_t b = __ballot(head);
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int q = 0; q < kWaves; q++) {
+      if (q < w) before += wsum[q];
+      total += wsum[q];
+    }
+    if (head) {
+      uint32_t u = carry + before + (uint32_t)__popcll(b & lt_mask);
+      ukeys[u] = keys[i];
+      upos[u] = i;
+    }
+    carry += total;
+    __syncthreads();
+  }
+}
+
+// counts[u] = next run start - this run start
+extern "C" __global__ void gqf_rle_counts(const uint64_t* __restrict__ upos, const uint64_t* __restrict__ m,
+                                          uint64_t n, uint64_t* __restrict__ counts) {
+  uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t mm = *m;
+  if (u >= mm) return;
+  uint64_t next = u + 1 < mm ? upos[u + 1] : n;
+  counts[u] = next - upos[u];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+static inline uint32_t sort_ntiles(uint64_t n) { return (uint32_t)((n + kTile - 1) / kTile); }
+
+uint64_t gqf_sort_hist_words(uint64_t n) { return (uint64_t)kMaxRadix * (sort_ntiles(n) ? sort_ntiles(n) : 1); }
+
+void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits,
+                     uint32_t* hist, uint32_t* totals, hipStream_t s) {
+  if (n == 0) return;
+  // an even number of passes leaves the result in `keys`
+  int passes, digit;
+  if (bits <= 40) {
+    passes = 4;
+    digit = (bits + 3) / 4;
+    if (digit < 1) digit = 1;
+  } else {
+    passes = 8;
+    digit = 8;
+  }
+  const uint32_t ntiles = sort_ntiles(n);
+  const int radix = 1 << digit;
+  uint64_t* in = keys;
+  uint64_t* out = tmp;
+  for (int p = 0; p < passes; p++) {
+    const int shift = p * digit;
+    const int h = (hash_it && p == 0) ? 1 : 0;
+    hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n, shift, digit, h, mask, hist,
+                       ntiles);
+    hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
+    hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
+    hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)in, out, n, shift,
+                       digit, h, mask, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+    uint64_t* t = in;
+    in = out;
+    out = t;
+  }
+}
+
+void gqf_launch_rle(const uint64_t* keys, uint64_t n, uint32_t* tile_heads, uint64_t* ukeys, uint64_t* upos,
+                    uint64_t* ucounts, uint64_t* m_dev, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t ntiles = sort_ntiles(n);
+  hipLaunchKernelGGL(gqf_rle_count, dim3(ntiles), dim3(kThreads), 0, s, keys, n, tile_heads);
+  hipLaunchKernelGGL(gqf_rle_scan, dim3(1), dim3(kThreads), 0, s, tile_heads, ntiles, m_dev);
+  hipLaunchKernelGGL(gqf_rle_emit, dim3(ntiles), dim3(kThreads), 0, s, keys, n, (const uint32_t*)tile_heads, ukeys,
+                     upos);
+  const uint32_t g = (uint32_t)((n + 255) / 256);
+  hipLaunchKernelGGL(gqf_rle_counts, dim3(g), dim3(256), 0, s, (const uint64_t*)upos, (const uint64_t*)m_dev, n,
+                     ucounts);
+}

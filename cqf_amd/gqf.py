@@ -1,0 +1,201 @@
+"""Python mirror of the GQF bulk API over libgqf's C ABI.
+
+The names and argument meaning follow the reference API
+(PASSIONLab/cqf include/gqf.cuh:87-140): `bulk_insert`, `bulk_insert_reduce`
+and `bulk_delete` overwrite the key tensor with the ascending hashes,
+`bulk_get` leaves `vals` unchanged and fills `returns`.  Device memory is
+held in torch tensors (dtype int64 carries the uint64 bit patterns); torch
+is only the allocator and stream owner here -- every operation runs in the
+HIP kernels of libgqf.so, and importing this module fails loudly when that
+library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgqf.so")
+
+QF_NO_LOCK = 0x01
+QF_TRY_ONCE_LOCK = 0x02
+QF_WAIT_FOR_LOCK = 0x04
+QF_KEY_IS_HASH = 0x08
+
+QF_STATUS_NO_SPACE = 0x1
+QF_STATUS_CLUSTER_OVERFLOW = 0x2
+QF_STATUS_BAD_HASH = 0x4
+QF_STATUS_POINT_FULL = 0x8
+
+# every extern "C" entry point declared in include/gqf.h
+EXPORTS = [
+    "qf_malloc_device", "qf_destroy_device",
+    "bulk_insert", "bulk_insert_reduce", "bulk_delete", "bulk_get", "bulk_get_misses_wrapper",
+    "point_get_wrapper", "point_bulk_insert_wrapper",
+    "qf_get_info", "qf_get_status", "qf_clear_status",
+    "qf_export_image", "qf_import_image", "qf_export_image_device", "qf_import_image_device",
+    "qf_reserve", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
+    "bulk_get_async", "qf_last_error",
+]
+
+
+class GqfInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "nslots", "xnslots", "nblocks", "key_bits", "key_remainder_bits", "bits_per_slot",
+        "range", "num_locks", "nregions", "total_size_in_bytes")]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+
+
+def load_library(path=LIB_PATH):
+    """Load libgqf.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libgqf.so not built at {path}: run `python -m cqf_amd.build`")
+    L = ctypes.CDLL(path)
+    L.qf_malloc_device.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_bool]
+    L.qf_destroy_device.argtypes = [_vp]
+    for fn in ("bulk_insert", "bulk_insert_reduce", "bulk_delete"):
+        getattr(L, fn).argtypes = [_vp, _u64, _vp, ctypes.c_uint8]
+    for fn in ("bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async"):
+        getattr(L, fn).argtypes = [_vp, _u64, _vp, ctypes.c_uint8, _vp]
+    L.bulk_get.argtypes = [_vp, _u64, _vp, _vp]
+    L.bulk_get_async.argtypes = [_vp, _u64, _vp, _vp, _vp]
+    L.bulk_get_misses_wrapper.argtypes = [_vp, _vp, _u64]
+    L.bulk_get_misses_wrapper.restype = _u64
+    L.point_get_wrapper.argtypes = [_vp, _vp, _u64]
+    L.point_get_wrapper.restype = _u64
+    L.point_bulk_insert_wrapper.argtypes = [_vp, _vp, _u64]
+    L.qf_get_info.argtypes = [_vp, ctypes.POINTER(GqfInfo)]
+    L.qf_get_info.restype = ctypes.c_int
+    L.qf_get_status.argtypes = [_vp]
+    L.qf_get_status.restype = ctypes.c_uint32
+    L.qf_clear_status.argtypes = [_vp]
+    L.qf_export_image.argtypes = [_vp, _vp, _u64]
+    L.qf_export_image.restype = ctypes.c_int
+    L.qf_import_image.argtypes = [_vp, _vp, _u64]
+    L.qf_import_image.restype = ctypes.c_int
+    L.qf_export_image_device.argtypes = [_vp, _vp, _u64, _vp]
+    L.qf_export_image_device.restype = ctypes.c_int
+    L.qf_import_image_device.argtypes = [_vp, _vp, _u64, _vp]
+    L.qf_import_image_device.restype = ctypes.c_int
+    L.qf_reserve.argtypes = [_vp, _u64]
+    L.qf_reserve.restype = ctypes.c_int
+    L.qf_last_error.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def to_device_u64(arr, device="cuda"):
+    """numpy uint64 -> torch int64 device tensor holding the same bits."""
+    import torch
+    a = np.ascontiguousarray(arr, dtype=np.uint64).view(np.int64)
+    return torch.from_numpy(a.copy()).to(device)
+
+
+def to_numpy_u64(t):
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+class GQF:
+    """One device-resident counting quotient filter (2^nbits slots, r = 8)."""
+
+    def __init__(self, nbits, bulk_config=True):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        self._lib.qf_malloc_device(ctypes.byref(h), nbits, bulk_config)
+        if not h.value:
+            raise RuntimeError(f"qf_malloc_device failed for nbits={nbits}")
+        self.handle = h
+        self.nbits = nbits
+        info = GqfInfo()
+        self._lib.qf_get_info(self.handle, ctypes.byref(info))
+        self.info = {n: int(getattr(info, n)) for n, _ in GqfInfo._fields_}
+
+    # -- lifecycle
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            self._lib.qf_destroy_device(self.handle)
+        self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def range(self):
+        return self.info["range"]
+
+    @property
+    def nbytes(self):
+        return self.info["total_size_in_bytes"]
+
+    # -- bulk API (tensors are int64 device tensors)
+    def bulk_insert(self, keys, flags=0, stream=None):
+        self._lib.bulk_insert_async(self.handle, keys.numel(), _ptr(keys), flags, _stream_ptr(stream))
+
+    def bulk_insert_reduce(self, keys, flags=0, stream=None):
+        self._lib.bulk_insert_reduce_async(self.handle, keys.numel(), _ptr(keys), flags, _stream_ptr(stream))
+
+    def bulk_delete(self, keys, flags=0, stream=None):
+        self._lib.bulk_delete_async(self.handle, keys.numel(), _ptr(keys), flags, _stream_ptr(stream))
+
+    def bulk_get(self, vals, returns=None, stream=None):
+        import torch
+        if returns is None:
+            returns = torch.empty_like(vals)
+        self._lib.bulk_get_async(self.handle, vals.numel(), _ptr(vals), _ptr(returns), _stream_ptr(stream))
+        return returns
+
+    def bulk_get_misses(self, vals):
+        return int(self._lib.bulk_get_misses_wrapper(self.handle, _ptr(vals), vals.numel()))
+
+    def point_get_misses(self, keys):
+        return int(self._lib.point_get_wrapper(self.handle, _ptr(keys), keys.numel()))
+
+    def point_bulk_insert(self, keys):
+        self._lib.point_bulk_insert_wrapper(self.handle, _ptr(keys), keys.numel())
+
+    def reserve(self, max_batch):
+        return self._lib.qf_reserve(self.handle, max_batch)
+
+    # -- status and image
+    def status(self):
+        return int(self._lib.qf_get_status(self.handle))
+
+    def clear_status(self):
+        self._lib.qf_clear_status(self.handle)
+
+    def export_image(self):
+        buf = np.empty(self.nbytes, dtype=np.uint8)
+        rc = self._lib.qf_export_image(self.handle, buf.ctypes.data_as(ctypes.c_void_p), buf.size)
+        if rc != 0:
+            raise RuntimeError(f"qf_export_image rc={rc}")
+        return buf
+
+    def import_image(self, img):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        rc = self._lib.qf_import_image(self.handle, img.ctypes.data_as(ctypes.c_void_p), img.size)
+        if rc != 0:
+            raise RuntimeError(f"qf_import_image rc={rc}")
+
+
+def last_error():
+    return int(load_library().qf_last_error())

@@ -1,0 +1,117 @@
+/*
+ * gqf.h -- C ABI of libgqf, a counting quotient filter for one AMD MI355X
+ * (gfx950).  Drop-in for the GQF API of PASSIONLab/cqf (include/gqf.cuh).
+ *
+ * Signatures, names and semantics of the reference entry points are kept:
+ *   - QF* is an opaque DEVICE pointer owned by the library;
+ *   - keys / vals / returns are DEVICE arrays owned by the caller;
+ *   - bulk_insert, bulk_insert_reduce and bulk_delete overwrite `keys` with
+ *     the ascending hashes (src/gqf.cu:3180-3181, 3206-3209);
+ *   - bulk calls are asynchronous on the null stream (src/gqf.cu:3184-3419);
+ *     the *_wrapper calls synchronise and return a host value.
+ * Differences, all additive: nvals == 0 is a no-op (the reference underflows
+ * its grid size); table-full and cluster-overflow conditions are recorded
+ * in a status word (qf_get_status) instead of being printed and ignored;
+ * stream-taking variants and image export/import exist for tests and the
+ * serving loop.  No signature carries a HIP or torch type: streams are
+ * passed as void* (a hipStream_t, or NULL for the null stream).
+ */
+#ifndef GQF_H
+#define GQF_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct quotient_filter quotient_filter;
+typedef quotient_filter QF;
+
+/* include/gqf.cuh:45-49 */
+enum qf_hashmode { QF_HASH_DEFAULT, QF_HASH_INVERTIBLE, QF_HASH_NONE };
+/* include/gqf.cuh:52-56 */
+enum qf_returns { QF_ITEM_INSERTED, QF_ITEM_FOUND, QF_FULL };
+
+/* include/gqf.cuh:74-80 */
+#define QF_NO_LOCK (0x01)
+#define QF_TRY_ONCE_LOCK (0x02)
+#define QF_WAIT_FOR_LOCK (0x04)
+#define QF_KEY_IS_HASH (0x08)
+
+/* include/gqf.cuh:213-215 */
+#define QF_NO_SPACE (-1)
+#define QF_COULDNT_LOCK (-2)
+#define QF_DOESNT_EXIST (-3)
+
+/* status bits (qf_get_status) */
+#define QF_STATUS_NO_SPACE 0x1u
+#define QF_STATUS_CLUSTER_OVERFLOW 0x2u
+#define QF_STATUS_BAD_HASH 0x4u
+#define QF_STATUS_POINT_FULL 0x8u
+
+/* ---- lifecycle ----------------------------------------------------- */
+
+/* replaces include/gqf.cuh:87 (src/gqf.cu:2123-2221): 2^nbits slots, 8-bit
+ * remainders, invertible hash, seed 0.  bulk_config = false also allocates
+ * the point-API region locks. */
+void qf_malloc_device(QF **qf, int nbits, bool bulk_config);
+/* replaces include/gqf.cuh:88 (src/gqf.cu:2225-2269) */
+void qf_destroy_device(QF *qf);
+
+/* ---- bulk API ------------------------------------------------------ */
+
+/* replaces include/gqf.cuh:113 (src/gqf.cu:3184-3232) */
+void bulk_insert(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags);
+/* replaces include/gqf.cuh:116 (src/gqf.cu:3238-3314) */
+void bulk_insert_reduce(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags);
+/* replaces include/gqf.cuh:119 (src/gqf.cu:3316-3363) */
+void bulk_delete(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags);
+/* replaces include/gqf.cuh:122 (src/gqf.cu:3397-3401); vals unchanged */
+void bulk_get(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns);
+/* replaces include/gqf.cuh:140 (src/gqf.cu:3404-3419): keys with count < 1 */
+uint64_t bulk_get_misses_wrapper(QF *qf, uint64_t *vals, uint64_t nvals);
+
+/* ---- point API, host side ------------------------------------------ */
+
+/* replaces include/gqf.cuh:134 (src/gqf.cu:3112-3150): point_query of
+ * every key, returns the number not found */
+uint64_t point_get_wrapper(QF *qf, uint64_t *hashes, uint64_t nitems);
+/* replaces the point_bulk_insert kernel launch of include/gqf.cuh:136
+ * (src/gqf.cu:3153-3162; launched by include/point_wrapper.cuh:63): one
+ * locked point_insert per key. */
+void point_bulk_insert_wrapper(QF *qf, uint64_t *hashes, uint64_t nitems);
+
+/* ---- additions (no reference counterpart) -------------------------- */
+
+typedef struct gqf_info {
+  uint64_t nslots, xnslots, nblocks, key_bits, key_remainder_bits, bits_per_slot;
+  uint64_t range, num_locks, nregions, total_size_in_bytes;
+} gqf_info;
+
+int qf_get_info(QF *qf, gqf_info *out);
+/* OR of QF_STATUS_* bits since the last clear (synchronises) */
+uint32_t qf_get_status(QF *qf);
+void qf_clear_status(QF *qf);
+/* copy the 81-byte block image (total_size_in_bytes) to / from host memory */
+int qf_export_image(QF *qf, void *host_dst, uint64_t nbytes);
+int qf_import_image(QF *qf, const void *host_src, uint64_t nbytes);
+/* copy the block image to / from device memory (stream-ordered) */
+int qf_export_image_device(QF *qf, void *dev_dst, uint64_t nbytes, void *stream);
+int qf_import_image_device(QF *qf, const void *dev_src, uint64_t nbytes, void *stream);
+/* size the per-filter workspace for batches up to max_batch keys so that
+ * later bulk calls allocate nothing */
+int qf_reserve(QF *qf, uint64_t max_batch);
+/* stream-ordered variants of the bulk calls */
+void bulk_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
+void bulk_insert_reduce_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
+void bulk_delete_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
+void bulk_get_async(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns, void *stream);
+/* last HIP error seen by the library (0 = none); clears it */
+int qf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GQF_H */
