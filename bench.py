@@ -1,0 +1,251 @@
+#!/usr/bin/env python
+"""Benchmark of the GQF bulk path on one MI355X.
+
+Workload (one step): an empty table of 2^28 slots (8-bit remainders) is
+filled to 90% load with bulk_insert in batches of 2^24 keys
+(splitmix64 seed 3, 241,591,910 keys), then bulk_get runs over the inserted
+keys and over 2^28 disjoint keys (outputs N .. N+2^28-1 of the same stream,
+distinct from the inserted ones because splitmix64 is a bijection of its
+state).  Keys are generated on the GPU and resident in HBM before the timed
+calls start; each step re-copies them (bulk_insert overwrites its keys with
+the sorted hashes) and empties the table.
+
+Prints one JSON line.  `value` is (inserts + queries) / (summed time of the
+bulk calls), in Mops/s; the separate insert and query rates, p50/p99 batch
+latencies, the roofline of the dominant kernel and the CPU baseline (the
+oracle's restatement of the reference host path, one core) ride along.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+INSERT_BYTES = 170  # 8 B key + 81 B home block read + 81 B write
+QUERY_BYTES = 97    # 8 B key + 81 B home block + 8 B count
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nbits", type=int, default=28)
+    ap.add_argument("--batch-log2", type=int, default=24)
+    ap.add_argument("--load", type=float, default=0.9)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--cpu-nbits", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_bulk_get_q28.md"),
+                    help="rocprofv3 counter summary holding the measured HBM bytes per launch")
+    return ap.parse_args()
+
+
+def read_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed counter summary
+    (a line `traffic <kernel> <bytes>`), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.strip().strip("|").split()
+                if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(nbits):
+    """Oracle (reference host path restated in C) on the reference's CPU
+    configuration: q = nbits, 90% load, splitmix64 seed 1 inserts, then the
+    same keys and 2^nbits disjoint seed-2 keys queried; one core."""
+    from oracle import oracle as O
+    n = int(0.9 * (1 << nbits))
+    keys = O.splitmix64(1, n)
+    q = O.splitmix64(2, 1 << nbits)
+    import numpy as np
+    qk = np.concatenate([keys, q])
+    t_ins, t_q, hits = O.time_insert_query(nbits, keys, qk)
+    ops = n + qk.size
+    return {
+        "value": round(ops / (t_ins + t_q) / 1e6, 3),
+        "unit": "Mops/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"q={nbits} table, {n} inserts to 90% load then {qk.size} queries "
+                   f"(reference host path restated in C, single thread)"),
+        "inserts_mops": round(n / t_ins / 1e6, 3),
+        "queries_mops": round(qk.size / t_q / 1e6, 3),
+        "seconds": round(t_ins + t_q, 3),
+        "cpu_model": platform.processor() or platform.machine(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(filters shard by hash prefix or by filter; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cqf_amd import gqf
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    nbits = args.nbits
+    n_ins = int(args.load * (1 << nbits))
+    batch = 1 << args.batch_log2
+    n_dis = 1 << nbits
+
+    f = gqf.GQF(nbits)
+    f.reserve(batch)
+    master = torch.empty(n_ins, dtype=torch.int64, device=dev)
+    work = torch.empty_like(master)
+    disjoint = torch.empty(n_dis, dtype=torch.int64, device=dev)
+    ret_ins = torch.empty_like(master)
+    ret_dis = torch.empty_like(disjoint)
+    gqf.fill_splitmix64(master, args.seed, 0, stream)
+    gqf.fill_splitmix64(disjoint, args.seed, n_ins, stream)
+    torch.cuda.synchronize()
+    bounds = list(range(0, n_ins, batch)) + [n_ins]
+
+    def step(record):
+        f.reset(stream)
+        work.copy_(master)
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(bounds) + 2)]
+        evs[0].record(stream)
+        for i in range(len(bounds) - 1):
+            f.bulk_insert(work[bounds[i]:bounds[i + 1]], 0, stream)
+            evs[i + 1].record(stream)
+        f.bulk_get(master, ret_ins, stream)
+        evs[-2].record(stream)
+        f.bulk_get(disjoint, ret_dis, stream)
+        evs[-1].record(stream)
+        if record is not None:
+            record.append(evs)
+
+    for _ in range(args.warmup):
+        step(None)
+    torch.cuda.synchronize()
+
+    gqf.profile_enable(True)
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(recs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ins_ms, q_ms, batch_lat, query_lat = 0.0, 0.0, [], []
+    nb = len(bounds) - 1
+    for evs in recs:
+        for i in range(nb):
+            batch_lat.append(evs[i].elapsed_time(evs[i + 1]))
+        ins_ms += evs[0].elapsed_time(evs[nb])
+        query_lat.append(evs[nb].elapsed_time(evs[nb + 1]))
+        query_lat.append(evs[nb + 1].elapsed_time(evs[nb + 2]))
+        q_ms += evs[nb].elapsed_time(evs[nb + 2])
+
+    # kernel-level times from the library's own events
+    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "region_bounds", "rebuild_phase",
+                                              "rebuild_wide", "bulk_get")}
+    gqf.profile_enable(False)
+
+    # correctness guard on the measured state: every inserted key present,
+    # false-positive rate near the closed form (distinct inserted)/2^(q+8)
+    present_ok = bool((ret_ins > 0).all().item())
+    fp = int((ret_dis > 0).sum().item())
+    status = f.status()
+
+    # exact keys per rebuild launch: work holds each batch's sorted hashes
+    even = 0
+    for i in range(nb):
+        h = work[bounds[i]:bounds[i + 1]]
+        even += int(((((h >> 14) // 48) & 1) == 0).sum().item())
+    keys_phase = (even, n_ins - even)  # insert keys handled by even / odd launches
+
+    steps = args.steps
+    n_q = n_ins + n_dis
+    ins_rate = n_ins * steps / (ins_ms / 1e3)
+    q_rate = n_q * steps / (q_ms / 1e3)
+    value = (n_ins + n_q) * steps / ((ins_ms + q_ms) / 1e3) / 1e6
+
+    # roofline of the dominant kernel (largest summed time)
+    cands = {}
+    ms_get, c_get = kern["bulk_get"]
+    if c_get:
+        cands["bulk_get"] = (ms_get, c_get, QUERY_BYTES * n_q * steps)
+    ms_rb, c_rb = kern["rebuild_phase"]
+    if c_rb:
+        cands["rebuild_phase"] = (ms_rb, c_rb, INSERT_BYTES * n_ins * steps)
+    dom = max(cands, key=lambda k: cands[k][0])
+    d_ms, d_cnt, d_bytes = cands[dom]
+    achieved = d_bytes / (d_ms / 1e3) / 1e9
+    traffic = read_traffic(args.traffic, dom)
+
+    lat = np.array(batch_lat)
+    qlat = np.array(query_lat)
+    out = {
+        "metric": "Minserts/s & Mqueries/s at 2^28 slots, 90% load; p50/p99 batch latency",
+        "value": round(value, 3),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"GQF bulk API: 2^{nbits} slots, r=8, bulk_insert {n_ins} splitmix64(seed={args.seed}) "
+                         f"keys in batches of 2^{args.batch_log2}, then bulk_get of the inserted keys and of "
+                         f"2^{nbits} disjoint keys"),
+            "nbits": nbits, "load": args.load, "batch": batch, "inserts": n_ins,
+            "queries": n_q, "seed": args.seed, "disjoint": f"splitmix64(seed={args.seed}) outputs {n_ins}..",
+        },
+        "inserts_mops": round(ins_rate / 1e6, 3),
+        "queries_mops": round(q_rate / 1e6, 3),
+        "insert_batch_ms_p50": round(float(np.percentile(lat, 50)), 4),
+        "insert_batch_ms_p99": round(float(np.percentile(lat, 99)), 4),
+        "query_call_ms_p50": round(float(np.percentile(qlat, 50)), 4),
+        "query_call_ms_p99": round(float(np.percentile(qlat, 99)), 4),
+        "value_basis": "sum of bulk-call times (hipEvents); ms_per_step also holds the table reset and key re-copy",
+        "insert_roofline_frac": round(ins_rate * INSERT_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+        "query_roofline_frac": round(q_rate * QUERY_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels_ms": {k: round(v[0] / steps, 3) for k, v in kern.items()},
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "launches": d_cnt,
+            "avg_launch_ms": round(d_ms / d_cnt, 4),
+            "bytes_per_unit": QUERY_BYTES if dom == "bulk_get" else INSERT_BYTES,
+        },
+        "checks": {"all_inserted_found": present_ok, "fp_hits": fp,
+                   "fp_rate": round(fp / n_dis, 6), "status": status,
+                   "rebuild_keys_even_odd": keys_phase},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_nbits)
+    f.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -11,7 +11,9 @@
 #include <string.h>
 
 #include <mutex>
+#include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "../../include/gqf.h"
 #include "gqf_internal.h"
@@ -364,6 +366,17 @@ extern "C" int qf_import_image_device(QF* qf, const void* dev_src, uint64_t nbyt
   return e == hipSuccess ? 0 : -3;
 }
 
+// Empties the table on the device (stream-ordered), as qf_reset empties the
+// host image (src/gqf.cu:2283-2299).
+extern "C" int qf_reset_device(QF* qf, void* stream) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const size_t bytes = (size_t)f->host.nblocks * GQF_BLOCK_BYTES;
+  hipError_t e = hipMemsetAsync(f->host.blocks, 0, bytes, (hipStream_t)stream);
+  note_error(e, "qf_reset_device");
+  return e == hipSuccess ? 0 : -3;
+}
+
 extern "C" int qf_reserve(QF* qf, uint64_t max_batch) {
   Filter* f = lookup(qf);
   if (!f) return -1;
@@ -376,4 +389,93 @@ extern "C" int qf_last_error(void) {
   int e = g_last_error;
   g_last_error = 0;
   return e;
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel event timing
+
+namespace {
+struct ProfRec {
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+std::unordered_map<std::string, size_t> g_prof_open;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  GQF_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void recycle_all() {
+  for (auto& r : g_prof) {
+    if (r.a) g_event_pool.push_back(r.a);
+    if (r.b) g_event_pool.push_back(r.b);
+  }
+  g_prof.clear();
+  g_prof_open.clear();
+}
+}  // namespace
+
+void gqf_prof_begin(const char* name, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return;
+  ProfRec r;
+  r.name = name;
+  r.a = take_event();
+  GQF_CHECK(hipEventRecord(r.a, s));
+  g_prof_open[r.name] = g_prof.size();
+  g_prof.push_back(r);
+}
+
+void gqf_prof_end(const char* name, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return;
+  auto it = g_prof_open.find(name);
+  if (it == g_prof_open.end()) return;
+  ProfRec& r = g_prof[it->second];
+  r.b = take_event();
+  GQF_CHECK(hipEventRecord(r.b, s));
+  g_prof_open.erase(it);
+}
+
+// Turns per-kernel event timing on (clearing earlier records) or off.
+extern "C" void gqf_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  recycle_all();
+  g_prof_on = on != 0;
+}
+
+// Sum of the completed launches of kernel `name` since the last enable:
+// total milliseconds and launch count.  Synchronises on the events.
+extern "C" int gqf_profile_read(const char* name, double* total_ms, uint64_t* launches) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  double t = 0;
+  uint64_t c = 0;
+  for (auto& r : g_prof) {
+    if (r.name != name || !r.b) continue;
+    GQF_CHECK(hipEventSynchronize(r.b));
+    float ms = 0;
+    GQF_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    t += ms;
+    c++;
+  }
+  if (total_ms) *total_ms = t;
+  if (launches) *launches = c;
+  return 0;
+}
+
+// Fills dev_out[0..n) with splitmix64 outputs skip .. skip+n-1 of `seed`.
+extern "C" void gqf_fill_splitmix64(uint64_t* dev_out, uint64_t n, uint64_t seed, uint64_t skip, void* stream) {
+  gqf_launch_splitmix64(dev_out, n, seed, skip, (hipStream_t)stream);
+  GQF_CHECK(hipGetLastError());
 }

@@ -15,9 +15,8 @@
 
 #define GQF_BLOCK_BYTES 81
 #define GQF_SLOTS_PER_BLOCK 64
-#define GQF_REGION_BLOCKS 64                       // one wave64 lane per block
+#define GQF_REGION_BLOCKS 48                       // rebuild region; + 16 push blocks = one wave64
 #define GQF_REGION_SLOTS (GQF_REGION_BLOCKS * GQF_SLOTS_PER_BLOCK)
-#define GQF_REGION_SHIFT 20                        // hash >> 20 = region (8 rem + 6 + 6)
 #define GQF_REF_LOCK_SLOTS 8192ULL                 // reference point-lock region (src/gqf.cu:58)
 #define GQF_REF_LOCK_DIST 64                       // reference lock stride (src/gqf.cu:59)
 #define GQF_OFFSET_SAT 255
@@ -46,6 +45,18 @@ struct quotient_filter {
   uint32_t* status;             // GQF_STATUS_* bits, sticky until cleared
 };
 typedef struct quotient_filter QF;
+
+// Pointers read out of the descriptor are generic; these casts let the
+// compiler emit global_load / global_store (not flat) for table accesses.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GQF_GLOBAL __attribute__((address_space(1)))
+#else
+#define GQF_GLOBAL
+#endif
+template <typename T>
+__device__ static inline const GQF_GLOBAL T* gqf_gptr(const T* p) { return (const GQF_GLOBAL T*)p; }
+template <typename T>
+__device__ static inline GQF_GLOBAL T* gqf_gptr_mut(T* p) { return (GQF_GLOBAL T*)p; }
 
 // ---------------------------------------------------------------------------
 // scalar helpers (host and device)
@@ -113,7 +124,7 @@ __device__ static inline uint32_t gqf_align_bytes(uint32_t hi, uint32_t lo, uint
 
 __device__ static inline gqf_header gqf_load_header(const uint8_t* blocks, uint64_t b) {
   uint64_t addr = b * GQF_BLOCK_BYTES;
-  const uint32_t* p = (const uint32_t*)(blocks + (addr & ~3ULL));
+  const GQF_GLOBAL uint32_t* p = gqf_gptr((const uint32_t*)(blocks + (addr & ~3ULL)));
   uint32_t s = (uint32_t)(addr & 3ULL);
   uint32_t d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3], d4 = p[4], d5 = p[5];
   uint32_t a0 = gqf_align_bytes(d1, d0, s), a1 = gqf_align_bytes(d2, d1, s);

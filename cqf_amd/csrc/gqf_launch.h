@@ -28,3 +28,11 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
 void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uint64_t* returns, hipStream_t s);
 void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsigned long long* misses,
                        hipStream_t s);
+
+void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t skip, hipStream_t s);
+
+// Per-kernel event timing (off by default).  When enabled, the launchers
+// bracket each kernel with hipEvents on its stream; gqf_profile_read sums
+// the completed intervals of one kernel name.
+void gqf_prof_begin(const char* name, hipStream_t s);
+void gqf_prof_end(const char* name, hipStream_t s);

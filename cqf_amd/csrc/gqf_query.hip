@@ -89,7 +89,7 @@ __device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash) {
   int64_t start = prev_end + 1;
   if (start < (int64_t)q) start = (int64_t)q;
   const uint32_t len = (uint32_t)(end - start + 1);
-  const uint8_t* blocks = qf.blocks;
+  const GQF_GLOBAL uint8_t* blocks = gqf_gptr(qf.blocks);
   auto slot = [&](uint32_t k) -> uint64_t { return blocks[gqf_slot_addr((uint64_t)start + k)]; };
   uint32_t idx = 0;
   do {
@@ -139,11 +139,35 @@ static inline uint32_t probe_grid(uint64_t n) {
 
 void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uint64_t* returns, hipStream_t s) {
   if (n == 0) return;
+  gqf_prof_begin("bulk_get", s);
   hipLaunchKernelGGL(gqf_bulk_get_kernel, dim3(probe_grid(n)), dim3(256), 0, s, qf_dev, vals, n, returns);
+  gqf_prof_end("bulk_get", s);
 }
 
 void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsigned long long* misses,
                        hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(gqf_bulk_misses_kernel, dim3(probe_grid(n)), dim3(256), 0, s, qf_dev, vals, n, misses);
+}
+
+// ---------------------------------------------------------------------------
+// synthetic key streams for benchmarks and tests: out[i] = splitmix64 output
+// number (skip + i) of the stream seeded with `seed` (Steele, Lea & Flood
+// 2014; the state advances by the golden gamma, then two xor-shift-multiply
+// rounds)
+
+extern "C" __global__ void __launch_bounds__(256)
+gqf_splitmix64_kernel(uint64_t* __restrict__ out, uint64_t n, uint64_t seed, uint64_t skip) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (skip + i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    out[i] = z ^ (z >> 31);
+  }
+}
+
+void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t skip, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(gqf_splitmix64_kernel, dim3(probe_grid(n)), dim3(256), 0, s, out, n, seed, skip);
 }

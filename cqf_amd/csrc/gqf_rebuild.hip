@@ -2,63 +2,68 @@
 //
 // Replaces the reference's even-odd scheduler in which one *thread* owns an
 // 8192-slot region and calls qf_insert / qf_remove once per key
-// (src/gqf.cu:2759-2906, 3184-3363).  Here one wave64 owns a 64-block region
-// (4096 quotients, one lane per block) and applies ALL of the region's keys
-// in one pass:
+// (src/gqf.cu:2759-2906, 3184-3363).  Here one wave64 owns a region of 48
+// blocks plus a 16-block push window into the next region, one lane per
+// block, and applies ALL of the region's keys in one pass:
 //
-//   1. the region's blocks plus a window of the next region are staged from
-//      HBM into LDS with coalesced 16-byte loads and split into
-//      slots / occupieds / runends / offsets;
-//   2. each lane walks its block's runs (old layout from the stored offsets),
-//      merges the block's sorted (hash, count) items counter by counter and
-//      gets the new run lengths;
-//   3. a wave-wide max-plus scan places every run at
-//      max(quotient, end of previous run + 1);
-//   4. runs of the next region's first blocks are shifted (right on insert,
-//      left on delete) until the layout meets its old position again;
-//   5. the new bytes are written back with coalesced stores.
+//   1. the window's 81-byte blocks are copied HBM -> LDS with 16-byte loads
+//      (the region starts at 48*81*r bytes, a multiple of 16) and the
+//      region's sorted items are cached in LDS as 20-bit relative hashes;
+//   2. each lane walks its block's runs in the old layout (starting from the
+//      stored offset), merges the block's items counter by counter, and
+//      reduces the new run lengths to a max-plus aggregate (a, b): the
+//      block maps "end of the runs before it" x to max(a, x + b);
+//   3. a wave scan of the aggregates places every run at
+//      max(quotient, end of previous run + 1) -- the canonical CQF layout;
+//   4. the push into the next region's first blocks (a pure shift of their
+//      runs) ends at the first block whose end-before is unchanged;
+//   5. the new image is assembled in LDS and stored with 16-byte writes.
 //
-// The result is the canonical image the reference's sequential inserts and
-// removes produce (counters in encode_counter form, offsets saturated at 255),
+// The result is the image the reference's sequential inserts and removes
+// produce (counters in encode_counter form, offsets saturated at 255),
 // because a CQF image is a function of the multiset of hashes alone.
 //
-// Regions are processed in two phases (even, then odd), so a region may push
-// into the next region's first blocks without racing the wave that owns it.
-// A region whose push is not absorbed inside its window is flagged and
-// redone by a single-workgroup pass with a 4x wider window, in ascending
-// region order, after the phase.
+// Regions run in two phases (even, then odd) so a push never races the
+// wave that owns the next region.  A region whose push is not absorbed in
+// its window is flagged and redone by a single-workgroup pass with a wide
+// window, in ascending region order, after the phase.
 #include "gqf_internal.h"
 #include "gqf_launch.h"
 
 namespace {
 
-constexpr int kWinFast = 80;   // 64 region blocks + 16 push blocks
-constexpr int kWinWide = 256;  // fallback window
-constexpr int64_t kNegInf = INT64_MIN / 4;
+constexpr int R = GQF_REGION_BLOCKS;  // 48 region blocks
+constexpr int kWinFast = 64;          // + 16 push blocks
+constexpr int kWinWide = 256;         // fallback window (208 push blocks)
+constexpr int kItemCap = 512;         // items cached in LDS per region
+constexpr int32_t kNegInf = INT32_MIN / 4;
 
 enum RebuildMode : int { kInsert = 0, kDelete = 1 };
-
 enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 
 template <int W>
 struct Smem {
-  uint8_t slots_in[W * 64];
-  uint8_t slots_out[W * 64];
-  uint64_t occ_in[W];
+  // whole 64-chunk rows so staging can load and store branch-free
+  uint4 raw_in[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
+  uint4 raw_out[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
   uint64_t run_in[W];
-  uint64_t occ_out[W];
   uint64_t run_out[W];
-  int64_t xold[W + 1];   // end of runs with quotient < block start (>= start-1), old layout
-  int64_t xnew[W + 1];   // same, new layout
-  int64_t agg_a[W];      // per-block max-plus aggregate of the new run lengths
-  int64_t agg_b[W];
-  uint32_t off_in[W];
+  int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
+  int32_t xnew[W + 1];
+  uint32_t items[kItemCap];  // hash - first hash of the region
+  uint32_t icnt[kItemCap];   // counts (reduce mode)
   uint32_t blk_cnt[64];
   uint32_t blk_start[64];
   int32_t first_invalid;
+  int32_t xend;
   int32_t bstar;
   int32_t result;
 };
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+// byte offset of window-relative slot s in the raw 81-byte image
+__device__ inline uint32_t raw_off(int32_t s) { return (uint32_t)(s >> 6) * GQF_BLOCK_BYTES + 17 + (s & 63); }
 
 // ---------------------------------------------------------------------------
 // global-image helpers (used when a stored offset is saturated)
@@ -101,29 +106,22 @@ __device__ int64_t true_x_global(const QF& qf, uint64_t b) {
   return x > lo ? x : lo;
 }
 
-// ---------------------------------------------------------------------------
-// wave helpers
-
-__device__ inline int lane_id() { return (int)__lane_id(); }
-
-__device__ inline void compose(int64_t& a, int64_t& b, int64_t a2, int64_t b2) {
+__device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   // (a, b) then (a2, b2): x -> max(a2, max(a, x + b) + b2)
-  int64_t na = a + b2;
+  int32_t na = a + b2;
   a = na > a2 ? na : a2;
   b = b + b2;
 }
 
-// ---------------------------------------------------------------------------
-// counter emission in slot order (same bytes as encode_counter,
+// counter emission in slot order (the bytes encode_counter writes,
 // src/gqf.cu:1216-1272)
-
 template <typename Put>
-__device__ inline uint32_t emit_counter(uint64_t rem, uint64_t count, Put put) {
+__device__ __forceinline__ uint32_t emit_counter(uint64_t rem, uint64_t count, Put put) {
   if (count == 0) return 0;
   if (count == 1) { put(0, rem); return 1; }
   if (count == 2) { put(0, rem); put(1, rem); return 2; }
-  if (count == 3) {
-    put(0, rem);  // r 0 r, or 0 0 0 for r == 0
+  if (count == 3) {  // r 0 r, or 0 0 0 for r == 0
+    put(0, rem);
     put(1, 0);
     put(2, rem);
     return 3;
@@ -139,73 +137,88 @@ __device__ inline uint32_t emit_counter(uint64_t rem, uint64_t count, Put put) {
   return k;
 }
 
+// The region's items: cached relative hashes in LDS, or (when the region
+// holds more than kItemCap) read from global memory.
+struct Items {
+  const uint64_t* __restrict__ g;
+  const uint64_t* __restrict__ gc;
+  uint64_t s;     // first item index of the region
+  uint64_t base;  // first hash of the region
+  bool cached;
+  __device__ inline uint32_t hash(const uint32_t* lds, uint32_t i) const {
+    return cached ? lds[i] : (uint32_t)(g[s + i] - base);
+  }
+  __device__ inline uint64_t count(const uint32_t* lds, uint32_t i) const {
+    if (!gc) return 1;
+    return cached ? (uint64_t)lds[i] : gc[s + i];
+  }
+};
+
 // ---------------------------------------------------------------------------
-// One block's walk: old runs (from the old layout) merged with the block's
-// items.  Pass 1 (WRITE = false) returns the max-plus aggregate of the new
-// run lengths and the new occupieds word; pass 2 (WRITE = true) lays the
-// runs out from x = xnew[j] into slots_out / run_out.
+// One block's walk: the old runs of block j merged with its items.  Pass 1
+// (WRITE = false) returns the block's max-plus aggregate and new occupieds
+// word; pass 2 (WRITE = true) writes the runs into raw_out from xnew[j].
 
 template <int W, bool WRITE>
-__device__ bool walk_block(const QF& qf, Smem<W>& sm, int j, int64_t win_base, int nwin,
-                           const uint64_t* __restrict__ items, const uint64_t* __restrict__ counts,
-                           uint64_t it, uint64_t ie, int mode,
-                           int64_t& agg_a, int64_t& agg_b, uint64_t& newocc) {
-  const int64_t blk_base = win_base + 64 * j;
-  int64_t xo = sm.xold[j];
-  int64_t xn = WRITE ? sm.xnew[j] : 0;
-  uint64_t occ = sm.occ_in[j];
+__device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const Items& src, uint32_t it, uint32_t ie,
+                           int mode, uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
+  const uint8_t* in = (const uint8_t*)sm.raw_in;
+  uint8_t* out = (uint8_t*)sm.raw_out;
+  const int32_t blk_base = 64 * j;
+  int32_t xo = sm.xold[j];
+  int32_t xn = WRITE ? sm.xnew[j] : 0;
   agg_a = kNegInf;
   agg_b = 0;
   newocc = 0;
+  int cw_blk = -1;
+  uint64_t cw = 0;
 
-  uint64_t cur_h = it < ie ? items[it] : 0;
+  uint32_t cur = it < ie ? src.hash(sm.items, it) : 0;
   for (;;) {
-    int qo = occ ? __builtin_ctzll(occ) : 64;
-    int qi = it < ie ? (int)((cur_h >> 8) & 63) : 64;
-    int qrel = qo < qi ? qo : qi;
+    const int qo = occ ? __builtin_ctzll(occ) : 64;
+    const int qi = it < ie ? (int)((cur >> 8) & 63) : 64;
+    const int qrel = qo < qi ? qo : qi;
     if (qrel == 64) break;
-    int64_t qa = blk_base + qrel;
+    const int32_t qa = blk_base + qrel;
 
-    // old run of qa
-    int64_t st_o = 0, en_o = -1;
+    // old run of qa: from max(qa, xo + 1) to the next run end
+    int32_t st_o = 0, en_o = -1;
     if (qo == qrel) {
       occ &= occ - 1;
       st_o = qa > xo + 1 ? qa : xo + 1;
-      int64_t rel = st_o - win_base;
-      int bj = (int)(rel >> 6);
+      int bj = st_o >> 6;
       if (bj >= nwin) return false;
-      uint64_t w = sm.run_in[bj] >> (rel & 63);
-      int64_t pos = rel;
+      if (bj != cw_blk) { cw_blk = bj; cw = sm.run_in[bj]; }
+      uint64_t w = cw >> (st_o & 63);
+      int32_t pos = st_o;
       while (!w) {
         bj++;
         if (bj >= nwin) return false;
-        w = sm.run_in[bj];
-        pos = (int64_t)bj * 64;
+        cw_blk = bj;
+        cw = sm.run_in[bj];
+        w = cw;
+        pos = 64 * bj;
       }
-      en_o = win_base + pos + __builtin_ctzll(w);
+      en_o = pos + __builtin_ctzll(w);
       xo = en_o;
     }
     const uint32_t old_len = (uint32_t)(en_o - st_o + 1);  // 0 if no old run
 
-    int64_t st_n = 0;
+    int32_t st_n = 0;
     if (WRITE) st_n = qa > xn + 1 ? qa : xn + 1;
     uint32_t len = 0;
 
     if (qi != qrel) {
       // untouched run: same bytes at the new position
       len = old_len;
-      if (WRITE) {
-        const int64_t src = st_o - win_base, dst = st_n - win_base;
-        for (uint32_t k = 0; k < len; k++) sm.slots_out[dst + k] = sm.slots_in[src + k];
-      }
+      if (WRITE)
+        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[raw_off(st_o + (int32_t)k)];
     } else {
       // two-pointer merge of the old counters of qa (ascending remainders)
       // with the items whose quotient is qa (ascending hashes)
-      const int64_t src = st_o - win_base;
-      const int64_t dst = WRITE ? st_n - win_base : 0;
-      auto old_slot = [&](uint32_t k) -> uint64_t { return sm.slots_in[src + k]; };
+      auto old_slot = [&](uint32_t k) -> uint64_t { return in[raw_off(st_o + (int32_t)k)]; };
       auto put = [&](uint32_t k, uint64_t v) {
-        if (WRITE) sm.slots_out[dst + len + k] = (uint8_t)v;
+        if (WRITE) out[raw_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
       };
       uint32_t oi = 0, o_end = 0;
       uint64_t o_rem = 0, o_cnt = 0;
@@ -214,15 +227,15 @@ __device__ bool walk_block(const QF& qf, Smem<W>& sm, int j, int64_t win_base, i
       uint64_t n_rem = 0, n_cnt = 0;
       bool have_new = false;
       auto next_new = [&]() {
-        have_new = it < ie && (int64_t)(cur_h >> 8) == qa;
+        have_new = it < ie && (int32_t)(cur >> 8) == qa;
         if (!have_new) return;
-        const uint64_t h0 = cur_h;
+        const uint32_t h0 = cur;
         n_rem = h0 & 0xFF;
         n_cnt = 0;
-        while (it < ie && cur_h == h0) {
-          n_cnt += counts ? counts[it] : 1ULL;
+        while (it < ie && cur == h0) {
+          n_cnt += src.count(sm.icnt, it);
           it++;
-          cur_h = it < ie ? items[it] : 0;
+          cur = it < ie ? src.hash(sm.items, it) : 0;
         }
       };
       auto next_old = [&]() {
@@ -235,11 +248,12 @@ __device__ bool walk_block(const QF& qf, Smem<W>& sm, int j, int64_t win_base, i
         if (have_old && (!have_new || o_rem < n_rem)) {
           const uint32_t clen = o_end - oi + 1;  // unchanged counter: copy its bytes
           if (WRITE)
-            for (uint32_t k = 0; k < clen; k++) sm.slots_out[dst + len + k] = sm.slots_in[src + oi + k];
+            for (uint32_t k = 0; k < clen; k++)
+              out[raw_off(st_n + (int32_t)(len + k))] = in[raw_off(st_o + (int32_t)(oi + k))];
           len += clen;
           next_old();
         } else if (!have_old || n_rem < o_rem) {
-          if (mode == kInsert) len += emit_counter(n_rem, n_cnt, put);  // delete of an absent key: no-op
+          if (mode == kInsert) len += emit_counter(n_rem, n_cnt, put);  // deleting an absent key: no-op
           next_new();
         } else {
           const uint64_t c = mode == kInsert ? o_cnt + n_cnt : (o_cnt > n_cnt ? o_cnt - n_cnt : 0);
@@ -252,12 +266,10 @@ __device__ bool walk_block(const QF& qf, Smem<W>& sm, int j, int64_t win_base, i
 
     if (len) {
       newocc |= 1ULL << qrel;
-      int64_t a2 = qa + (int64_t)len - 1;
-      compose(agg_a, agg_b, a2, (int64_t)len);
+      compose(agg_a, agg_b, qa + (int32_t)len - 1, (int32_t)len);
       if (WRITE) {
-        int64_t en_n = st_n + (int64_t)len - 1;
-        int64_t rel = en_n - win_base;
-        atomicOr((unsigned long long*)&sm.run_out[rel >> 6], 1ULL << (rel & 63));
+        const int32_t en_n = st_n + (int32_t)len - 1;
+        atomicOr((unsigned long long*)&sm.run_out[en_n >> 6], 1ULL << (en_n & 63));
         xn = en_n;
       }
     }
@@ -265,54 +277,94 @@ __device__ bool walk_block(const QF& qf, Smem<W>& sm, int j, int64_t win_base, i
   return true;
 }
 
+// header of window block j from the raw LDS image
+__device__ inline void read_header(const uint8_t* raw, int j, uint32_t& off, uint64_t& occ, uint64_t& run) {
+  const uint8_t* p = raw + j * GQF_BLOCK_BYTES;
+  uint64_t o = 0, r = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    o |= (uint64_t)p[1 + k] << (8 * k);
+    r |= (uint64_t)p[9 + k] << (8 * k);
+  }
+  off = p[0];
+  occ = o;
+  run = r;
+}
+
+__device__ inline void write_header(uint8_t* raw, int j, uint32_t off, uint64_t occ, uint64_t run) {
+  uint8_t* p = raw + j * GQF_BLOCK_BYTES;
+  p[0] = (uint8_t)off;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    p[1 + k] = (uint8_t)(occ >> (8 * k));
+    p[9 + k] = (uint8_t)(run >> (8 * k));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Region rebuild.  All 64 lanes of the (single-wave) workgroup call it.
+// Lane l handles window blocks l, l + 64, ...
 
 template <int W>
-__device__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
+__device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
                               Smem<W>& sm) {
+  constexpr int kRounds = W / 64;
+  constexpr int kChunks = (W * GQF_BLOCK_BYTES + 15) / 16;
+  constexpr int kChunkIters = (kChunks + 63) / 64;
   const int lane = lane_id();
-  const uint64_t B0 = r * GQF_REGION_BLOCKS;
+  const uint64_t B0 = r * R;
   const int nwin = (int)min((uint64_t)W, qf.nblocks - B0);
-  const int nreg = (int)min((uint64_t)GQF_REGION_BLOCKS, qf.nblocks - B0);
+  const int nreg = (int)min((uint64_t)R, qf.nblocks - B0);
   const int64_t win_base = (int64_t)(B0 * 64);
   const bool table_end_in_window = B0 + (uint64_t)nwin >= qf.nblocks;
+  const uint32_t m = (uint32_t)(e - s);
+  const Items src{items, counts, s, B0 << 14, m <= (uint32_t)kItemCap};
 
-  // ---- 1. stage the window into LDS (coalesced 16-byte loads) ----
+  // ---- 1. stage the window and the items ----
   {
-    const uint8_t* src = qf.blocks + B0 * GQF_BLOCK_BYTES;  // 16-byte aligned: 5184 r
-    const uint32_t nbytes = (uint32_t)nwin * GQF_BLOCK_BYTES;
-    const uint32_t nchunks = (nbytes + 15) / 16;
-    for (uint32_t c = lane; c < nchunks; c += 64) {
-      uint4 v = ((const uint4*)src)[c];
-      uint32_t words[4] = {v.x, v.y, v.z, v.w};
+    // 16-byte aligned (3888 r); every lane issues all its loads (clamped
+    // in bounds) before the first LDS write, so they are in flight together
+    const GQF_GLOBAL uint4* g = gqf_gptr((const uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
+    const uint32_t nchunks = ((uint32_t)nwin * GQF_BLOCK_BYTES + 15) / 16;
+    uint4 v[kChunkIters];
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        uint32_t o = c * 16 + k;
-        if (o >= nbytes) break;
-        uint8_t byte = (uint8_t)(words[k >> 2] >> (8 * (k & 3)));
-        uint32_t j = o / GQF_BLOCK_BYTES;
-        uint32_t f = o - j * GQF_BLOCK_BYTES;
-        if (f == 0) sm.off_in[j] = byte;
-        else if (f < 9) ((uint8_t*)sm.occ_in)[j * 8 + (f - 1)] = byte;
-        else if (f < 17) ((uint8_t*)sm.run_in)[j * 8 + (f - 9)] = byte;
-        else sm.slots_in[j * 64 + (f - 17)] = byte;
+    for (int k = 0; k < kChunkIters; k++) {
+      const uint32_t c = (uint32_t)lane + 64u * k;
+      v[k] = g[c < nchunks ? c : nchunks - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kChunkIters; k++) sm.raw_in[lane + 64 * k] = v[k];
+    if (src.cached) {
+      for (uint32_t i = lane; i < m; i += 64) {
+        sm.items[i] = (uint32_t)(items[s + i] - src.base);
+        if (counts) sm.icnt[i] = (uint32_t)counts[s + i];
       }
     }
-    for (int j = lane; j < 64; j += 64) { sm.blk_cnt[j] = 0; }
+    sm.blk_cnt[lane] = 0;
     if (lane == 0) { sm.first_invalid = nwin; sm.bstar = -1; sm.result = kOk; }
   }
   __syncthreads();
 
-  // ---- 2. per-block item ranges (items are sorted) ----
-  for (uint64_t i = s + lane; i < e; i += 64) {
-    uint32_t j = (uint32_t)((items[i] >> 14) - B0);
-    atomicAdd(&sm.blk_cnt[j], 1u);
+  // ---- 2. headers, old end-before per block, per-block item ranges ----
+  uint64_t occ_w[kRounds];
+#pragma unroll
+  for (int k = 0; k < kRounds; k++) {
+    const int j = lane + 64 * k;
+    occ_w[k] = 0;
+    if (j < nwin) {
+      uint32_t off;
+      uint64_t run;
+      read_header((const uint8_t*)sm.raw_in, j, off, occ_w[k], run);
+      sm.run_in[j] = run;
+      sm.xold[j] = off < GQF_OFFSET_SAT ? 64 * j - 1 + (int32_t)off
+                                        : (int32_t)(true_x_global(qf, B0 + j) - win_base);
+    }
   }
+  for (uint32_t i = lane; i < m; i += 64) atomicAdd(&sm.blk_cnt[src.hash(sm.items, i) >> 14], 1u);
   __syncthreads();
   {
-    uint32_t c = sm.blk_cnt[lane];
+    const uint32_t c = sm.blk_cnt[lane];
     uint32_t incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -321,128 +373,101 @@ __device__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restri
     }
     sm.blk_start[lane] = incl - c;
   }
-
-  // ---- 3. old end-before for every window block (true offsets) ----
-  for (int j = lane; j < nwin; j += 64) {
-    uint32_t o = sm.off_in[j];
-    int64_t bstart = win_base + 64 * j;
-    sm.xold[j] = o < GQF_OFFSET_SAT ? bstart - 1 + (int64_t)o : true_x_global(qf, B0 + j);
-  }
   __syncthreads();
 
-  // ---- 4. pass 1: new run lengths per block ----
-  for (int j = lane; j < nwin; j += 64) {
-    uint64_t it = 0, ie = 0;
-    if (j < nreg) {
-      it = s + sm.blk_start[j];
-      ie = it + sm.blk_cnt[j];
-    }
-    int64_t a, b;
-    uint64_t occn;
-    bool ok = walk_block<W, false>(qf, sm, j, win_base, nwin, items, counts, it, ie, mode, a, b, occn);
-    sm.agg_a[j] = a;
-    sm.agg_b[j] = b;
-    sm.occ_out[j] = occn;
-    if (!ok) atomicMin(&sm.first_invalid, j);
-  }
-  __syncthreads();
-
-  // ---- 5. max-plus scan over blocks -> xnew ----
+  // ---- 3. pass 1 + max-plus scan -> xnew ----
+  uint64_t occ_new[kRounds];
   {
-    const int64_t e_in = sm.xold[0];
-    int64_t ca = kNegInf, cb = 0;  // carry over previous chunks
-    for (int base = 0; base < nwin; base += 64) {
-      int j = base + lane;
-      int64_t a = kNegInf, b = 0;
-      if (j < nwin) { a = sm.agg_a[j]; b = sm.agg_b[j]; }
-      // inclusive scan within the wave
-      int64_t ia = a, ib = b;
+    const int32_t e_in = sm.xold[0];
+    int32_t ca = kNegInf, cb = 0;  // carry over earlier rounds
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      const int j = lane + 64 * k;
+      int32_t a = kNegInf, b = 0;
+      occ_new[k] = 0;
+      if (j < nwin) {
+        uint32_t it = 0, ie = 0;
+        if (j < nreg) {
+          it = sm.blk_start[j];
+          ie = it + sm.blk_cnt[j];
+        }
+        const bool ok = walk_block<W, false>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occ_new[k]);
+        if (!ok) atomicMin(&sm.first_invalid, j);
+      }
+      int32_t ia = a, ib = b;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
-        int64_t pa = __shfl_up(ia, d, 64);
-        int64_t pb = __shfl_up(ib, d, 64);
+        const int32_t pa = __shfl_up(ia, d, 64);
+        const int32_t pb = __shfl_up(ib, d, 64);
         if (lane >= d) {
-          int64_t na = pa + ib;
+          const int32_t na = pa + ib;
           ia = na > ia ? na : ia;
           ib = pb + ib;
         }
       }
-      // exclusive = inclusive of lane-1, composed after the carry
-      int64_t ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64);
+      int32_t ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64);
       if (lane == 0) { ea = kNegInf; eb = 0; }
-      int64_t pa = ca, pb = cb;
+      int32_t pa = ca, pb = cb;
       compose(pa, pb, ea, eb);
       if (j < nwin) {
-        int64_t x = pa > e_in + pb ? pa : e_in + pb;
-        int64_t lo = win_base + 64 * j - 1;
-        sm.xnew[j] = x > lo ? x : lo;
+        const int32_t x = pa > e_in + pb ? pa : e_in + pb;
+        sm.xnew[j] = x > 64 * j - 1 ? x : 64 * j - 1;
       }
-      int64_t ta = __shfl(ia, 63, 64), tb = __shfl(ib, 63, 64);
-      compose(ca, cb, ta, tb);
+      compose(ca, cb, __shfl(ia, 63, 64), __shfl(ib, 63, 64));
     }
     if (lane == 0) {
-      int64_t x = ca > e_in + cb ? ca : e_in + cb;
-      int64_t lo = win_base + 64 * nwin - 1;
-      sm.xnew[nwin] = x > lo ? x : lo;
+      const int32_t x = ca > e_in + cb ? ca : e_in + cb;
+      sm.xnew[nwin] = x > 64 * nwin - 1 ? x : 64 * nwin - 1;
+      sm.xend = x;  // true end of the last run in the window (not normalised)
     }
   }
   __syncthreads();
 
-  // ---- 6. where does the new layout meet the old one? ----
+  // ---- 4. where does the new layout meet the old one? ----
   {
     const int first_invalid = sm.first_invalid;
-    int cand = nwin;  // nwin = "no meeting block inside the window"
-    for (int base = GQF_REGION_BLOCKS; base < nwin; base += 64) {
-      int j = base + lane;
-      bool ok = j < nwin && j <= first_invalid && sm.xnew[j] == sm.xold[j] &&
-                sm.xold[j] - (win_base + 64 * j - 1) < 64;
-      uint64_t m = __ballot(ok);
-      if (m) { cand = base + __builtin_ctzll(m); break; }
+    int cand = nwin;
+    for (int base = R; base < nwin; base += 64) {
+      const int j = base + lane;
+      const bool ok = j < nwin && j <= first_invalid && sm.xnew[j] == sm.xold[j] &&
+                      sm.xold[j] - (64 * j - 1) < 64;
+      const uint64_t bm = __ballot(ok);
+      if (bm) { cand = base + __builtin_ctzll(bm); break; }
     }
     if (lane == 0) {
       int res = kOk;
-      int bstar = cand;
-      if (cand == nwin) {
-        if (table_end_in_window && first_invalid == nwin) {
-          // the table ends inside the window: everything up to the end is rewritten
-          if (sm.xnew[nwin] >= (int64_t)qf.xnslots) res = kNoSpace;
-        } else {
-          res = kNeedWide;
-        }
-      } else {
-        // runs placed before the meeting block must end inside the table
-        if (sm.xnew[cand] >= (int64_t)qf.xnslots) res = kNoSpace;
-      }
-      // the spilled-in runs of the previous region must fit the window
-      if (sm.xold[0] - win_base + 1 > (int64_t)nwin * 64) res = kNeedWide;
-      sm.bstar = bstar;
+      const int64_t xn_end = win_base + (cand < nwin ? sm.xnew[cand] : sm.xend);
+      if (cand == nwin && !(table_end_in_window && first_invalid == nwin)) res = kNeedWide;
+      else if (xn_end >= (int64_t)qf.xnslots) res = kNoSpace;
+      if (sm.xold[0] + 1 > 64 * nwin) res = kNeedWide;  // spill-in longer than the window
+      sm.bstar = cand;
       sm.result = res;
     }
   }
   __syncthreads();
   const int res = sm.result;
   if (res != kOk) return res;
-  const int bstar = sm.bstar;                 // blocks [0, bstar) re-laid, bstar keeps its suffix
+  const int bstar = sm.bstar;  // blocks [0, bstar) re-laid out, bstar keeps its own runs
   const int nwrite = bstar < nwin ? bstar + 1 : nwin;
+  const uint32_t wbytes = (uint32_t)nwrite * GQF_BLOCK_BYTES;
 
-  // ---- 7. pass 2: lay out the runs ----
-  for (int i = lane; i < nwrite * 64; i += 64) sm.slots_out[i] = 0;
+  // ---- 5. pass 2: assemble the new image in LDS ----
+  for (uint32_t c = lane; c < (wbytes + 15) / 16; c += 64) sm.raw_out[c] = make_uint4(0, 0, 0, 0);
   for (int j = lane; j < nwrite; j += 64) sm.run_out[j] = 0;
   __syncthreads();
   {
-    // slots of the previous region's runs spilling into this one are kept
-    const int64_t e_in_rel = sm.xold[0] - win_base;  // last spilled slot (relative), may be -1
-    for (int64_t i = lane; i <= e_in_rel; i += 64) {
-      sm.slots_out[i] = sm.slots_in[i];
+    const uint8_t* in = (const uint8_t*)sm.raw_in;
+    uint8_t* out = (uint8_t*)sm.raw_out;
+    // the previous region's runs spilling into this window are kept
+    for (int32_t i = lane; i <= sm.xold[0]; i += 64) {
+      out[raw_off(i)] = in[raw_off(i)];
       if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
         atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
     }
     // the meeting block keeps everything after the new end-before
     if (bstar < nwin) {
-      const int64_t from = sm.xnew[bstar] - win_base + 1;
-      const int64_t to = (int64_t)(bstar + 1) * 64;
-      for (int64_t i = from + lane; i < to; i += 64) {
-        sm.slots_out[i] = sm.slots_in[i];
+      for (int32_t i = sm.xnew[bstar] + 1 + lane; i < 64 * (bstar + 1); i += 64) {
+        out[raw_off(i)] = in[raw_off(i)];
         if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
           atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
       }
@@ -450,57 +475,48 @@ __device__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restri
   }
   __syncthreads();
   const int nrelay = bstar < nwin ? bstar : nwin;
-  for (int j = lane; j < nrelay; j += 64) {
-    uint64_t it = 0, ie = 0;
-    if (j < nreg) {
-      it = s + sm.blk_start[j];
-      ie = it + sm.blk_cnt[j];
+#pragma unroll
+  for (int k = 0; k < kRounds; k++) {
+    const int j = lane + 64 * k;
+    if (j < nrelay) {
+      uint32_t it = 0, ie = 0;
+      if (j < nreg) {
+        it = sm.blk_start[j];
+        ie = it + sm.blk_cnt[j];
+      }
+      int32_t a, b;
+      uint64_t occn;
+      walk_block<W, true>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occn);
     }
-    int64_t a, b;
-    uint64_t occn;
-    walk_block<W, true>(qf, sm, j, win_base, nwin, items, counts, it, ie, mode, a, b, occn);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRounds; k++) {
+    const int j = lane + 64 * k;
+    if (j < nwrite) {
+      uint32_t off;
+      uint64_t occ;
+      if (j == bstar) {
+        uint64_t run;
+        read_header((const uint8_t*)sm.raw_in, j, off, occ, run);
+      } else {
+        const int32_t o = sm.xnew[j] - (64 * j - 1);
+        off = o > GQF_OFFSET_SAT ? GQF_OFFSET_SAT : (uint32_t)o;
+        occ = occ_new[k];
+      }
+      write_header((uint8_t*)sm.raw_out, j, off, occ, sm.run_out[j]);
+    }
   }
   __syncthreads();
 
-  // ---- 8. store blocks [0, nwrite) (coalesced 16-byte chunks) ----
+  // ---- 6. store blocks [0, nwrite) ----
   {
-    uint8_t* dst = qf.blocks + B0 * GQF_BLOCK_BYTES;
-    const uint32_t nbytes = (uint32_t)nwrite * GQF_BLOCK_BYTES;
-    const uint32_t nchunks = (nbytes + 15) / 16;
-    for (uint32_t c = lane; c < nchunks; c += 64) {
-      uint32_t words[4] = {0, 0, 0, 0};
-      uint32_t valid = 0;
-#pragma unroll
-      for (int k = 0; k < 16; k++) {
-        uint32_t o = c * 16 + k;
-        if (o >= nbytes) break;
-        uint32_t j = o / GQF_BLOCK_BYTES;
-        uint32_t f = o - j * GQF_BLOCK_BYTES;
-        uint32_t byte;
-        if (f == 0) {
-          if ((int)j == bstar) {
-            byte = sm.off_in[j];
-          } else {
-            int64_t off = sm.xnew[j] - (win_base + 64 * (int64_t)j - 1);
-            byte = off > GQF_OFFSET_SAT ? GQF_OFFSET_SAT : (uint32_t)off;
-          }
-        } else if (f < 9) {
-          uint64_t w = (int)j == bstar ? sm.occ_in[j] : sm.occ_out[j];
-          byte = (uint32_t)(w >> (8 * (f - 1))) & 0xFFu;
-        } else if (f < 17) {
-          byte = (uint32_t)(sm.run_out[j] >> (8 * (f - 9))) & 0xFFu;
-        } else {
-          byte = sm.slots_out[j * 64 + (f - 17)];
-        }
-        words[k >> 2] |= byte << (8 * (k & 3));
-        valid = k + 1;
-      }
-      if (valid == 16) {
-        ((uint4*)dst)[c] = make_uint4(words[0], words[1], words[2], words[3]);
-      } else {
-        for (uint32_t k = 0; k < valid; k++) dst[c * 16 + k] = (uint8_t)(words[k >> 2] >> (8 * (k & 3)));
-      }
-    }
+    GQF_GLOBAL uint4* g = gqf_gptr_mut((uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
+    const uint32_t full = wbytes / 16;
+    for (uint32_t c = lane; c < full; c += 64) g[c] = sm.raw_out[c];
+    const uint32_t tail = wbytes - full * 16;
+    if ((uint32_t)lane < tail)
+      ((GQF_GLOBAL uint8_t*)g)[full * 16 + lane] = ((const uint8_t*)sm.raw_out)[full * 16 + lane];
   }
   return kOk;
 }
@@ -521,7 +537,7 @@ gqf_rebuild_phase(const QF* __restrict__ qfp, const uint64_t* __restrict__ items
   if (r >= qf.nregions) return;
   const uint64_t s = region_start[r], e = region_start[r + 1];
   if (s == e) return;
-  int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm);
+  const int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm);
   if (lane_id() == 0) {
     if (res == kNeedWide) {
       wide_flags[r] = 1;
@@ -544,15 +560,15 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   const QF qf = *qfp;
   const int lane = lane_id();
   for (uint64_t base = (uint64_t)parity; base < qf.nregions; base += 128) {
-    uint64_t r = base + 2ULL * lane;
-    bool f = r < qf.nregions && wide_flags[r];
+    const uint64_t r = base + 2ULL * lane;
+    const bool f = r < qf.nregions && wide_flags[r];
     uint64_t m = __ballot(f);
     while (m) {
-      int l = __builtin_ctzll(m);
+      const int l = __builtin_ctzll(m);
       m &= m - 1;
-      uint64_t rr = base + 2ULL * l;
+      const uint64_t rr = base + 2ULL * l;
       const uint64_t s = region_start[rr], e = region_start[rr + 1];
-      int res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm);
+      const int res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm);
       __syncthreads();
       if (lane == 0) {
         wide_flags[rr] = 0;
@@ -565,32 +581,23 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   if (lane == 0) *wide_count = 0;
 }
 
-// region_start[r] = first index i with items[i] >> 20 >= r, for r in
-// [0, nregions]; items at or above `range` fall past the last region.
+// region_start[r] = first index i with items[i] >= r * (48 * 64 * 256), for
+// r in [0, nregions]; items at or above `range` fall past the last region.
 extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                                              uint64_t n_host, const uint64_t* __restrict__ n_dev,
                                              uint32_t* __restrict__ region_start) {
   const QF qf = *qfp;
-  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r > qf.nregions) return;
   const uint64_t n = n_dev ? *n_dev : n_host;
   uint64_t lo = 0, hi = n;
-  if (r == qf.nregions) {
-    // first item at or above the table's hash range
-    while (lo < hi) {
-      uint64_t mid = (lo + hi) >> 1;
-      if (items[mid] < qf.range) lo = mid + 1; else hi = mid;
-    }
-    region_start[r] = (uint32_t)lo;
-    if (lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
-    return;
-  }
-  const uint64_t bound = r << GQF_REGION_SHIFT;
+  const uint64_t bound = r == qf.nregions ? qf.range : (r * R) << 14;
   while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
+    const uint64_t mid = (lo + hi) >> 1;
     if (items[mid] < bound) lo = mid + 1; else hi = mid;
   }
   region_start[r] = (uint32_t)lo;
+  if (r == qf.nregions && lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,7 +606,9 @@ extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const u
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, hipStream_t s) {
   const uint32_t g = (uint32_t)((nregions + 1 + 255) / 256);
+  gqf_prof_begin("region_bounds", s);
   hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(256), 0, s, qf_dev, items, n, n_dev, region_start);
+  gqf_prof_end("region_bounds", s);
 }
 
 void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* items, const uint64_t* counts,
@@ -607,9 +616,13 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
                         hipStream_t s) {
   const uint32_t g = (uint32_t)((nregions + 1) / 2);
   for (int parity = 0; parity < 2; parity++) {
+    gqf_prof_begin("rebuild_phase", s);
     hipLaunchKernelGGL(gqf_rebuild_phase, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start, parity,
                        mode, wide_flags, wide_count);
+    gqf_prof_end("rebuild_phase", s);
+    gqf_prof_begin("rebuild_wide", s);
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(1), dim3(64), 0, s, qf_dev, items, counts, region_start, parity, mode,
                        wide_flags, wide_count);
+    gqf_prof_end("rebuild_wide", s);
   }
 }

@@ -283,7 +283,7 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
   if (bits <= 40) {
     passes = 4;
     digit = (bits + 3) / 4;
-    if (digit < 1) digit = 1;
+    if (digit < 6) digit = 6;
   } else {
     passes = 8;
     digit = 8;
@@ -295,12 +295,14 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
   for (int p = 0; p < passes; p++) {
     const int shift = p * digit;
     const int h = (hash_it && p == 0) ? 1 : 0;
+    gqf_prof_begin("sort_pass", s);
     hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n, shift, digit, h, mask, hist,
                        ntiles);
     hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
     hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
     hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)in, out, n, shift,
                        digit, h, mask, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+    gqf_prof_end("sort_pass", s);
     uint64_t* t = in;
     in = out;
     out = t;

@@ -34,8 +34,8 @@ EXPORTS = [
     "point_get_wrapper", "point_bulk_insert_wrapper",
     "qf_get_info", "qf_get_status", "qf_clear_status",
     "qf_export_image", "qf_import_image", "qf_export_image_device", "qf_import_image_device",
-    "qf_reserve", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
-    "bulk_get_async", "qf_last_error",
+    "qf_reserve", "qf_reset_device", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
+    "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
 ]
 
 
@@ -84,9 +84,15 @@ def load_library(path=LIB_PATH):
     L.qf_export_image_device.restype = ctypes.c_int
     L.qf_import_image_device.argtypes = [_vp, _vp, _u64, _vp]
     L.qf_import_image_device.restype = ctypes.c_int
+    L.qf_reset_device.argtypes = [_vp, _vp]
+    L.qf_reset_device.restype = ctypes.c_int
     L.qf_reserve.argtypes = [_vp, _u64]
     L.qf_reserve.restype = ctypes.c_int
     L.qf_last_error.restype = ctypes.c_int
+    L.gqf_profile_enable.argtypes = [ctypes.c_int]
+    L.gqf_profile_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]
+    L.gqf_profile_read.restype = ctypes.c_int
+    L.gqf_fill_splitmix64.argtypes = [_vp, _u64, _u64, _u64, _vp]
     _lib = L
     return L
 
@@ -173,6 +179,9 @@ class GQF:
     def point_bulk_insert(self, keys):
         self._lib.point_bulk_insert_wrapper(self.handle, _ptr(keys), keys.numel())
 
+    def reset(self, stream=None):
+        return self._lib.qf_reset_device(self.handle, _stream_ptr(stream))
+
     def reserve(self, max_batch):
         return self._lib.qf_reserve(self.handle, max_batch)
 
@@ -195,6 +204,23 @@ class GQF:
         rc = self._lib.qf_import_image(self.handle, img.ctypes.data_as(ctypes.c_void_p), img.size)
         if rc != 0:
             raise RuntimeError(f"qf_import_image rc={rc}")
+
+
+def fill_splitmix64(out, seed, skip=0, stream=None):
+    """out (int64 device tensor) <- splitmix64 outputs skip.. of `seed`."""
+    load_library().gqf_fill_splitmix64(_ptr(out), out.numel(), seed, skip, _stream_ptr(stream))
+    return out
+
+
+def profile_enable(on=True):
+    load_library().gqf_profile_enable(1 if on else 0)
+
+
+def profile_read(name):
+    ms = ctypes.c_double()
+    cnt = _u64()
+    load_library().gqf_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+    return ms.value, int(cnt.value)
 
 
 def last_error():

@@ -100,6 +100,8 @@ int qf_import_image(QF *qf, const void *host_src, uint64_t nbytes);
 /* copy the block image to / from device memory (stream-ordered) */
 int qf_export_image_device(QF *qf, void *dev_dst, uint64_t nbytes, void *stream);
 int qf_import_image_device(QF *qf, const void *dev_src, uint64_t nbytes, void *stream);
+/* empty the table (stream-ordered); mirrors qf_reset, src/gqf.cu:2283-2299 */
+int qf_reset_device(QF *qf, void *stream);
 /* size the per-filter workspace for batches up to max_batch keys so that
  * later bulk calls allocate nothing */
 int qf_reserve(QF *qf, uint64_t max_batch);
@@ -110,6 +112,16 @@ void bulk_delete_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, vo
 void bulk_get_async(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns, void *stream);
 /* last HIP error seen by the library (0 = none); clears it */
 int qf_last_error(void);
+
+/* ---- measurement support (no reference counterpart) ---------------- */
+
+/* per-kernel hipEvent timing on the launch stream: enable (clears), then
+ * read the summed milliseconds and launch count of one kernel name:
+ * "sort_pass", "region_bounds", "rebuild_phase", "rebuild_wide", "bulk_get" */
+void gqf_profile_enable(int on);
+int gqf_profile_read(const char *name, double *total_ms, uint64_t *launches);
+/* dev_out[i] = splitmix64 output number skip+i of the stream seeded with seed */
+void gqf_fill_splitmix64(uint64_t *dev_out, uint64_t n, uint64_t seed, uint64_t skip, void *stream);
 
 #ifdef __cplusplus
 }
