@@ -35,7 +35,7 @@ namespace {
 constexpr int R = GQF_REGION_BLOCKS;  // 48 region blocks
 constexpr int kWinFast = 64;          // + 16 push blocks
 constexpr int kWinWide = 256;         // fallback window (208 push blocks)
-constexpr int kItemCap = 512;         // items cached in LDS per region
+constexpr int kItemCap = 320;         // items cached in LDS per region (fast path)
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
 enum RebuildMode : int { kInsert = 0, kDelete = 1 };
@@ -50,10 +50,19 @@ struct Smem {
   uint64_t run_out[W];
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
+  uint64_t occ_in[W];
   uint32_t items[kItemCap];  // hash - first hash of the region
   uint32_t icnt[kItemCap];   // counts (reduce mode)
+  // one entry per distinct quotient among the items (fast path)
+  uint16_t tq[kItemCap];     // window-relative quotient
+  uint16_t tit0[kItemCap];   // first item of the quotient
+  int16_t tst[kItemCap];     // old run start
+  uint16_t tol[kItemCap];    // old run length (0: quotient was empty)
+  uint16_t tnl[kItemCap];    // new run length
+  int16_t tsn[kItemCap];     // new run start
   uint32_t blk_cnt[64];
   uint32_t blk_start[64];
+  uint32_t nq;
   int32_t first_invalid;
   int32_t xend;
   int32_t bstar;
@@ -155,6 +164,69 @@ struct Items {
 };
 
 // ---------------------------------------------------------------------------
+// Two-pointer merge of the old counters of one quotient (ascending
+// remainders, old run at st_o .. st_o + old_len - 1 of raw_in) with its items
+// [it0, it1) (ascending hashes).  Returns the new run length; with WRITE the
+// counters are written from st_n into raw_out.  Insert adds the counts,
+// delete subtracts them (floored at 0; deleting an absent key is a no-op),
+// as sequences of the reference's qf_insert / qf_remove would
+// (src/gqf.cu:1523-1933).
+
+template <int W, bool WRITE>
+__device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
+                                                   int32_t st_o, uint32_t old_len, int32_t st_n, int mode) {
+  const uint8_t* in = (const uint8_t*)sm.raw_in;
+  uint8_t* out = (uint8_t*)sm.raw_out;
+  uint32_t len = 0;
+  auto old_slot = [&](uint32_t k) -> uint64_t { return in[raw_off(st_o + (int32_t)k)]; };
+  auto put = [&](uint32_t k, uint64_t v) {
+    if (WRITE) out[raw_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
+  };
+  uint32_t oi = 0, o_end = 0;
+  uint64_t o_rem = 0, o_cnt = 0;
+  bool have_old = old_len > 0;
+  if (have_old) o_end = gqf_decode_counter(old_slot, old_len, 0, &o_rem, &o_cnt);
+  uint64_t n_rem = 0, n_cnt = 0;
+  bool have_new = false;
+  uint32_t it = it0;
+  auto next_new = [&]() {
+    have_new = it < it1;
+    if (!have_new) return;
+    const uint32_t h0 = src.hash(sm.items, it);
+    n_rem = h0 & 0xFF;
+    n_cnt = 0;
+    while (it < it1 && src.hash(sm.items, it) == h0) {
+      n_cnt += src.count(sm.icnt, it);
+      it++;
+    }
+  };
+  auto next_old = [&]() {
+    oi = o_end + 1;
+    have_old = oi < old_len;
+    if (have_old) o_end = gqf_decode_counter(old_slot, old_len, oi, &o_rem, &o_cnt);
+  };
+  next_new();
+  while (have_old || have_new) {
+    if (have_old && (!have_new || o_rem < n_rem)) {
+      const uint32_t clen = o_end - oi + 1;  // unchanged counter: copy its bytes
+      if (WRITE)
+        for (uint32_t k = 0; k < clen; k++) out[raw_off(st_n + (int32_t)(len + k))] = in[raw_off(st_o + (int32_t)(oi + k))];
+      len += clen;
+      next_old();
+    } else if (!have_old || n_rem < o_rem) {
+      if (mode == kInsert) len += emit_counter(n_rem, n_cnt, put);
+      next_new();
+    } else {
+      const uint64_t c = mode == kInsert ? o_cnt + n_cnt : (o_cnt > n_cnt ? o_cnt - n_cnt : 0);
+      len += emit_counter(n_rem, c, put);
+      next_old();
+      next_new();
+    }
+  }
+  return len;
+}
+
+// ---------------------------------------------------------------------------
 // One block's walk: the old runs of block j merged with its items.  Pass 1
 // (WRITE = false) returns the block's max-plus aggregate and new occupieds
 // word; pass 2 (WRITE = true) writes the runs into raw_out from xnew[j].
@@ -214,56 +286,122 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       if (WRITE)
         for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[raw_off(st_o + (int32_t)k)];
     } else {
-      // two-pointer merge of the old counters of qa (ascending remainders)
-      // with the items whose quotient is qa (ascending hashes)
-      auto old_slot = [&](uint32_t k) -> uint64_t { return in[raw_off(st_o + (int32_t)k)]; };
-      auto put = [&](uint32_t k, uint64_t v) {
-        if (WRITE) out[raw_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
-      };
-      uint32_t oi = 0, o_end = 0;
-      uint64_t o_rem = 0, o_cnt = 0;
-      bool have_old = old_len > 0;
-      if (have_old) o_end = gqf_decode_counter(old_slot, old_len, 0, &o_rem, &o_cnt);
-      uint64_t n_rem = 0, n_cnt = 0;
-      bool have_new = false;
-      auto next_new = [&]() {
-        have_new = it < ie && (int32_t)(cur >> 8) == qa;
-        if (!have_new) return;
-        const uint32_t h0 = cur;
-        n_rem = h0 & 0xFF;
-        n_cnt = 0;
-        while (it < ie && cur == h0) {
-          n_cnt += src.count(sm.icnt, it);
-          it++;
-          cur = it < ie ? src.hash(sm.items, it) : 0;
-        }
-      };
-      auto next_old = [&]() {
-        oi = o_end + 1;
-        have_old = oi < old_len;
-        if (have_old) o_end = gqf_decode_counter(old_slot, old_len, oi, &o_rem, &o_cnt);
-      };
-      next_new();
-      while (have_old || have_new) {
-        if (have_old && (!have_new || o_rem < n_rem)) {
-          const uint32_t clen = o_end - oi + 1;  // unchanged counter: copy its bytes
-          if (WRITE)
-            for (uint32_t k = 0; k < clen; k++)
-              out[raw_off(st_n + (int32_t)(len + k))] = in[raw_off(st_o + (int32_t)(oi + k))];
-          len += clen;
-          next_old();
-        } else if (!have_old || n_rem < o_rem) {
-          if (mode == kInsert) len += emit_counter(n_rem, n_cnt, put);  // deleting an absent key: no-op
-          next_new();
-        } else {
-          const uint64_t c = mode == kInsert ? o_cnt + n_cnt : (o_cnt > n_cnt ? o_cnt - n_cnt : 0);
-          len += emit_counter(n_rem, c, put);
-          next_old();
-          next_new();
-        }
-      }
+      const uint32_t it0 = it;
+      while (it < ie && (int32_t)(src.hash(sm.items, it) >> 8) == qa) it++;
+      cur = it < ie ? src.hash(sm.items, it) : 0;
+      len = merge_quotient<W, WRITE>(sm, src, it0, it, st_o, old_len, st_n, mode);
     }
 
+    if (len) {
+      newocc |= 1ULL << qrel;
+      compose(agg_a, agg_b, qa + (int32_t)len - 1, (int32_t)len);
+      if (WRITE) {
+        const int32_t en_n = st_n + (int32_t)len - 1;
+        atomicOr((unsigned long long*)&sm.run_out[en_n >> 6], 1ULL << (en_n & 63));
+        xn = en_n;
+      }
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Fast path (all of a region's items cached in LDS).  The quotients the
+// items touch are listed once (table tq/tit0); each lane then locates the old
+// run of one listed quotient straight from its block's header -- the rank
+// of the quotient among the block's occupied quotients selects its run end
+// among the run ends after the block's end-before -- and merges it with the
+// quotient's items.  The per-block walk then only reads the table, so its
+// loop body is the same for every lane.
+
+template <int W>
+__device__ __forceinline__ bool locate_old_run(const Smem<W>& sm, int nwin, int32_t q, int32_t& st_o,
+                                               uint32_t& old_len) {
+  const int j = q >> 6;
+  const int bit = q & 63;
+  const uint64_t occ = sm.occ_in[j];
+  if (!((occ >> bit) & 1ULL)) {
+    st_o = q;
+    old_len = 0;
+    return true;
+  }
+  const int rank = __popcll(occ & ((2ULL << bit) - 1ULL));
+  int32_t prev = sm.xold[j], end = -1;
+  const int32_t pos = sm.xold[j] + 1;
+  int bj = pos >> 6;
+  if (bj >= nwin) return false;
+  uint64_t w = sm.run_in[bj] & (~0ULL << (pos & 63));
+  int seen = 0;
+  for (;;) {
+    const int c = __popcll(w);
+    if (rank >= 2 && rank - 2 >= seen && rank - 2 < seen + c) prev = 64 * bj + gqf_select64(w, rank - 2 - seen);
+    if (rank - 1 < seen + c) {
+      end = 64 * bj + gqf_select64(w, rank - 1 - seen);
+      break;
+    }
+    seen += c;
+    if (++bj >= nwin) return false;
+    w = sm.run_in[bj];
+  }
+  st_o = q > prev + 1 ? q : prev + 1;
+  old_len = (uint32_t)(end - st_o + 1);
+  return true;
+}
+
+template <int W, bool WRITE>
+__device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tp, uint32_t te,
+                                                 uint64_t occ, int32_t& agg_a, int32_t& agg_b,
+                                                 uint64_t& newocc) {
+  const uint8_t* in = (const uint8_t*)sm.raw_in;
+  uint8_t* out = (uint8_t*)sm.raw_out;
+  const int32_t blk_base = 64 * j;
+  int32_t xo = sm.xold[j];
+  int32_t xn = WRITE ? sm.xnew[j] : 0;
+  agg_a = kNegInf;
+  agg_b = 0;
+  newocc = 0;
+  int cw_blk = -1;
+  uint64_t cw = 0;
+  for (;;) {
+    const int qo = occ ? __builtin_ctzll(occ) : 64;
+    const int qt = tp < te ? (int)(sm.tq[tp] & 63) : 64;
+    const int qrel = qo < qt ? qo : qt;
+    if (qrel == 64) break;
+    const int32_t qa = blk_base + qrel;
+    int32_t st_n = 0;
+    if (WRITE) st_n = qa > xn + 1 ? qa : xn + 1;
+    uint32_t len;
+    if (qt == qrel) {
+      // touched quotient: lengths from the table; its counters are written later
+      len = sm.tnl[tp];
+      if (qo == qrel) {
+        occ &= occ - 1;
+        xo = (int32_t)sm.tst[tp] + (int32_t)sm.tol[tp] - 1;
+      }
+      if (WRITE) sm.tsn[tp] = (int16_t)st_n;
+      tp++;
+    } else {
+      occ &= occ - 1;
+      const int32_t st_o = qa > xo + 1 ? qa : xo + 1;
+      int bj = st_o >> 6;
+      if (bj >= nwin) return false;
+      if (bj != cw_blk) { cw_blk = bj; cw = sm.run_in[bj]; }
+      uint64_t w = cw >> (st_o & 63);
+      int32_t pos = st_o;
+      while (!w) {
+        bj++;
+        if (bj >= nwin) return false;
+        cw_blk = bj;
+        cw = sm.run_in[bj];
+        w = cw;
+        pos = 64 * bj;
+      }
+      const int32_t en_o = pos + __builtin_ctzll(w);
+      xo = en_o;
+      len = (uint32_t)(en_o - st_o + 1);
+      if (WRITE)
+        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[raw_off(st_o + (int32_t)k)];
+    }
     if (len) {
       newocc |= 1ULL << qrel;
       compose(agg_a, agg_b, qa + (int32_t)len - 1, (int32_t)len);
@@ -357,11 +495,33 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       uint64_t run;
       read_header((const uint8_t*)sm.raw_in, j, off, occ_w[k], run);
       sm.run_in[j] = run;
+      sm.occ_in[j] = occ_w[k];
       sm.xold[j] = off < GQF_OFFSET_SAT ? 64 * j - 1 + (int32_t)off
                                         : (int32_t)(true_x_global(qf, B0 + j) - win_base);
     }
   }
-  for (uint32_t i = lane; i < m; i += 64) atomicAdd(&sm.blk_cnt[src.hash(sm.items, i) >> 14], 1u);
+  const bool fast = src.cached;
+  if (fast) {
+    // list the distinct quotients of the items (sorted, so heads compact in order)
+    uint32_t carry = 0;
+    const uint64_t lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+    for (uint32_t base = 0; base < m; base += 64) {
+      const uint32_t i = base + lane;
+      const uint32_t h = i < m ? sm.items[i] : 0;
+      const bool head = i < m && (i == 0 || (sm.items[i - 1] >> 8) != (h >> 8));
+      const uint64_t hm = __ballot(head);
+      if (head) {
+        const uint32_t t = carry + (uint32_t)__popcll(hm & lt);
+        sm.tq[t] = (uint16_t)(h >> 8);
+        sm.tit0[t] = (uint16_t)i;
+        atomicAdd(&sm.blk_cnt[h >> 14], 1u);
+      }
+      carry += (uint32_t)__popcll(hm);
+    }
+    if (lane == 0) sm.nq = carry;
+  } else {
+    for (uint32_t i = lane; i < m; i += 64) atomicAdd(&sm.blk_cnt[src.hash(sm.items, i) >> 14], 1u);
+  }
   __syncthreads();
   {
     const uint32_t c = sm.blk_cnt[lane];
@@ -372,6 +532,27 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       if (lane >= d) incl += t;
     }
     sm.blk_start[lane] = incl - c;
+  }
+  if (fast) {
+    // old run and new length of every listed quotient, one quotient per lane
+    const uint32_t nq = sm.nq;
+    for (uint32_t t = lane; t < nq; t += 64) {
+      const int32_t q = sm.tq[t];
+      int32_t st_o;
+      uint32_t old_len;
+      uint32_t nl = 0;
+      if (locate_old_run(sm, nwin, q, st_o, old_len)) {
+        const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
+        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0, mode);
+      } else {
+        atomicMin(&sm.first_invalid, q >> 6);
+        st_o = q;
+        old_len = 0;
+      }
+      sm.tst[t] = (int16_t)st_o;
+      sm.tol[t] = (uint16_t)old_len;
+      sm.tnl[t] = (uint16_t)nl;
+    }
   }
   __syncthreads();
 
@@ -391,7 +572,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
           it = sm.blk_start[j];
           ie = it + sm.blk_cnt[j];
         }
-        const bool ok = walk_block<W, false>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occ_new[k]);
+        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, ie, occ_w[k], a, b, occ_new[k])
+                             : walk_block<W, false>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occ_new[k]);
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
       int32_t ia = a, ib = b;
@@ -486,7 +668,18 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       }
       int32_t a, b;
       uint64_t occn;
-      walk_block<W, true>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occn);
+      if (fast) walk_block_table<W, true>(sm, j, nwin, it, ie, occ_w[k], a, b, occn);
+      else walk_block<W, true>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occn);
+    }
+  }
+  __syncthreads();
+  if (fast) {
+    // counters of the listed quotients at their new starts
+    const uint32_t nq = sm.nq;
+    for (uint32_t t = lane; t < nq; t += 64) {
+      if (sm.tnl[t] == 0) continue;
+      const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
+      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t], mode);
     }
   }
   __syncthreads();
