@@ -1,0 +1,119 @@
+"""Pure-Python model of the CQF image, written from the format's definition.
+
+Independent of oracle/cqf_oracle.c: it knows nothing of insert1 / shifting;
+it builds the image a multiset of hashes must have from the invariants the
+reference's readers rely on (PASSIONLab/cqf):
+  - block layout: offset u8, occupieds u64 LE, runends u64 LE, 64 slot bytes
+    (include/gqf_int.cuh:38-57);
+  - counter format (the comment at src/gqf.cu:1207-1215):
+        1: x   2: x x   3: x 0 x (x > 0) / 0 0 0   >3: x [0] c..c x / 0 c..c 0 0
+    with digits c in base 2^8-2 (x > 0; values 1..255 skipping x) or 2^8-1
+    (x = 0; values 1..255), most significant first, and the leading 0 when
+    the top digit is >= x;
+  - the run of quotient q starts at max(q, end of previous run + 1) and holds
+    its remainders' counters in ascending remainder order;
+  - block i's offset is run_end(64 i - 1) - 64 i + 1 floored at 0, stored
+    saturated at 255.
+Used by the CPU tests to pin the oracle; slow, so only for small tables.
+"""
+
+
+def encode(rem, count):
+    """Slots of the counter (rem, count), in slot order, from the format."""
+    if count == 0:
+        return []
+    if count == 1:
+        return [rem]
+    if count == 2:
+        return [rem, rem]
+    if count == 3:
+        return [rem, 0, rem] if rem else [0, 0, 0]
+    if rem:
+        base, c = 254, count - 3
+    else:
+        base, c = 255, count - 4
+    digits = []
+    while True:
+        d = c % base + 1
+        if rem and d >= rem:
+            d += 1
+        digits.append(d)
+        c //= base
+        if c == 0:
+            break
+    digits.reverse()  # most significant first
+    if rem:
+        lead = [0] if digits[0] >= rem else []
+        return [rem] + lead + digits + [rem]
+    return [0] + digits + [0, 0]
+
+
+def sizes(nbits):
+    nslots = 1 << nbits
+    # the reference computes nslots + 10*sqrt(nslots) in double precision and
+    # truncates (src/gqf.cu:1950); for nslots = 2^k the product is exact
+    # enough that isqrt of 100*nslots reproduces it
+    import math
+    xnslots = nslots + math.isqrt(100 * nslots)
+    nblocks = (xnslots + 63) // 64
+    return nslots, xnslots, nblocks
+
+
+def build_image(nbits, multiset):
+    """multiset: dict hash -> count (hash < 2^(nbits+8)).  Returns bytes."""
+    nslots, xnslots, nblocks = sizes(nbits)
+    slots = [0] * (nblocks * 64)
+    runend = [0] * (nblocks * 64)
+    occ = [0] * (nblocks * 64)
+    by_q = {}
+    for h, c in multiset.items():
+        if c > 0:
+            by_q.setdefault(h >> 8, []).append((h & 0xFF, c))
+    prev_end = -1
+    ends = []  # (quotient, end) in quotient order
+    for q in sorted(by_q):
+        run = []
+        for rem, c in sorted(by_q[q]):
+            run += encode(rem, c)
+        start = max(q, prev_end + 1)
+        end = start + len(run) - 1
+        if end >= xnslots:
+            raise ValueError("does not fit")
+        slots[start:end + 1] = run
+        runend[end] = 1
+        occ[q] = 1
+        ends.append((q, end))
+        prev_end = end
+    img = bytearray(nblocks * 81)
+    k = 0
+    last_end = -1
+    for b in range(nblocks):
+        # end of the runs with quotient <= 64 b - 1
+        while k < len(ends) and ends[k][0] <= 64 * b - 1:
+            last_end = ends[k][1]
+            k += 1
+        off = max(0, last_end - (64 * b - 1)) if b > 0 else 0
+        base = b * 81
+        img[base] = min(off, 255)
+        o = sum(occ[64 * b + i] << i for i in range(64))
+        r = sum(runend[64 * b + i] << i for i in range(64))
+        img[base + 1:base + 9] = o.to_bytes(8, "little")
+        img[base + 9:base + 17] = r.to_bytes(8, "little")
+        img[base + 17:base + 81] = bytes(slots[64 * b:64 * b + 64])
+    return bytes(img)
+
+
+def hash_64(key, mask):
+    """Thomas Wang's 64-bit mix written with multiplications
+    ((key << 21) - key - 1, *265, *21, *(2^31 + 1)), not the reference's
+    shift-add form (src/hashutil.cu:132-142)."""
+    M = (1 << 64) - 1
+    key &= M
+    key = (((key << 21) - key - 1) & M) & mask
+    key ^= key >> 24
+    key = (key * 265) & M & mask
+    key ^= key >> 14
+    key = (key * 21) & M & mask
+    key ^= key >> 28
+    key = (key * ((1 << 31) + 1)) & M & mask
+    return key
