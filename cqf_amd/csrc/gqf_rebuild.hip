@@ -44,10 +44,14 @@ enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 template <int W>
 struct Smem {
   // whole 64-chunk rows so staging can load and store branch-free
-  uint4 raw_in[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
-  uint4 raw_out[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
+  // the raw 81-byte image: staged in here, then (once slots and headers are
+  // extracted) reused to assemble the output; whole 64-chunk rows so staging
+  // can load and store branch-free
+  uint4 raw[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
+  uint32_t sin[W * 17];  // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
   uint64_t run_in[W];
   uint64_t run_out[W];
+  uint64_t tmask[W];     // quotients of block j touched by the items (fast path)
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
   uint64_t occ_in[W];
@@ -73,6 +77,15 @@ __device__ inline int lane_id() { return (int)__lane_id(); }
 
 // byte offset of window-relative slot s in the raw 81-byte image
 __device__ inline uint32_t raw_off(int32_t s) { return (uint32_t)(s >> 6) * GQF_BLOCK_BYTES + 17 + (s & 63); }
+
+// byte offset of window-relative slot p in sin: one 68-byte row per block,
+// so lanes reading "their" block at the same column hit different banks
+__device__ inline uint32_t sin_off(int32_t p) { return (uint32_t)(p >> 6) * 68 + (p & 63); }
+
+// LDS dword read at a 4-byte aligned byte offset of a byte buffer
+__device__ inline uint32_t lds_dword(const uint8_t* base, uint32_t byte_off) {
+  return *(const uint32_t*)(base + byte_off);
+}
 
 // ---------------------------------------------------------------------------
 // global-image helpers (used when a stored offset is saturated)
@@ -165,9 +178,9 @@ struct Items {
 
 // ---------------------------------------------------------------------------
 // Two-pointer merge of the old counters of one quotient (ascending
-// remainders, old run at st_o .. st_o + old_len - 1 of raw_in) with its items
+// remainders, old run at st_o .. st_o + old_len - 1 of sin) with its items
 // [it0, it1) (ascending hashes).  Returns the new run length; with WRITE the
-// counters are written from st_n into raw_out.  Insert adds the counts,
+// counters are written from st_n into raw.  Insert adds the counts,
 // delete subtracts them (floored at 0; deleting an absent key is a no-op),
 // as sequences of the reference's qf_insert / qf_remove would
 // (src/gqf.cu:1523-1933).
@@ -175,10 +188,10 @@ struct Items {
 template <int W, bool WRITE>
 __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
                                                    int32_t st_o, uint32_t old_len, int32_t st_n, int mode) {
-  const uint8_t* in = (const uint8_t*)sm.raw_in;
-  uint8_t* out = (uint8_t*)sm.raw_out;
+  const uint8_t* in = (const uint8_t*)sm.sin;
+  uint8_t* out = (uint8_t*)sm.raw;
   uint32_t len = 0;
-  auto old_slot = [&](uint32_t k) -> uint64_t { return in[raw_off(st_o + (int32_t)k)]; };
+  auto old_slot = [&](uint32_t k) -> uint64_t { return in[sin_off(st_o + (int32_t)k)]; };
   auto put = [&](uint32_t k, uint64_t v) {
     if (WRITE) out[raw_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
   };
@@ -210,7 +223,7 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
     if (have_old && (!have_new || o_rem < n_rem)) {
       const uint32_t clen = o_end - oi + 1;  // unchanged counter: copy its bytes
       if (WRITE)
-        for (uint32_t k = 0; k < clen; k++) out[raw_off(st_n + (int32_t)(len + k))] = in[raw_off(st_o + (int32_t)(oi + k))];
+        for (uint32_t k = 0; k < clen; k++) out[raw_off(st_n + (int32_t)(len + k))] = in[sin_off(st_o + (int32_t)(oi + k))];
       len += clen;
       next_old();
     } else if (!have_old || n_rem < o_rem) {
@@ -229,13 +242,13 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
 // ---------------------------------------------------------------------------
 // One block's walk: the old runs of block j merged with its items.  Pass 1
 // (WRITE = false) returns the block's max-plus aggregate and new occupieds
-// word; pass 2 (WRITE = true) writes the runs into raw_out from xnew[j].
+// word; pass 2 (WRITE = true) writes the runs into raw from xnew[j].
 
 template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const Items& src, uint32_t it, uint32_t ie,
                            int mode, uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
-  const uint8_t* in = (const uint8_t*)sm.raw_in;
-  uint8_t* out = (uint8_t*)sm.raw_out;
+  const uint8_t* in = (const uint8_t*)sm.sin;
+  uint8_t* out = (uint8_t*)sm.raw;
   const int32_t blk_base = 64 * j;
   int32_t xo = sm.xold[j];
   int32_t xn = WRITE ? sm.xnew[j] : 0;
@@ -284,7 +297,7 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       // untouched run: same bytes at the new position
       len = old_len;
       if (WRITE)
-        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[raw_off(st_o + (int32_t)k)];
+        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
     } else {
       const uint32_t it0 = it;
       while (it < ie && (int32_t)(src.hash(sm.items, it) >> 8) == qa) it++;
@@ -349,61 +362,70 @@ __device__ __forceinline__ bool locate_old_run(const Smem<W>& sm, int nwin, int3
 }
 
 template <int W, bool WRITE>
-__device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tp, uint32_t te,
-                                                 uint64_t occ, int32_t& agg_a, int32_t& agg_b,
-                                                 uint64_t& newocc) {
-  const uint8_t* in = (const uint8_t*)sm.raw_in;
-  uint8_t* out = (uint8_t*)sm.raw_out;
+__device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tbase, uint64_t occ,
+                                                 int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
+  const uint8_t* in = (const uint8_t*)sm.sin;
+  uint8_t* out = (uint8_t*)sm.raw;
+  const uint64_t touched = sm.tmask[j];
   const int32_t blk_base = 64 * j;
   int32_t xo = sm.xold[j];
   int32_t xn = WRITE ? sm.xnew[j] : 0;
   agg_a = kNegInf;
   agg_b = 0;
   newocc = 0;
-  int cw_blk = -1;
-  uint64_t cw = 0;
-  for (;;) {
-    const int qo = occ ? __builtin_ctzll(occ) : 64;
-    const int qt = tp < te ? (int)(sm.tq[tp] & 63) : 64;
-    const int qrel = qo < qt ? qo : qt;
-    if (qrel == 64) break;
-    const int32_t qa = blk_base + qrel;
+  // run ends of this block's runs, as a bit stream from xo + 1 on: the k-th
+  // occupied quotient owns the k-th set bit, so each one is ctz + clear
+  int bj = (xo + 1) >> 6;
+  uint64_t w = 0;
+  if (occ) {
+    if (bj >= nwin) return false;
+    w = sm.run_in[bj] & (~0ULL << ((xo + 1) & 63));
+  }
+  uint64_t u = occ | touched;
+  while (u) {
+    const int q = __builtin_ctzll(u);
+    const uint64_t bit = 1ULL << q;
+    u ^= bit;
+    const int32_t qa = blk_base + q;
+    int32_t st_o = 0, en_o = -1;
+    if (occ & bit) {
+      while (!w) {
+        if (++bj >= nwin) return false;
+        w = sm.run_in[bj];
+      }
+      en_o = 64 * bj + __builtin_ctzll(w);
+      w &= w - 1;
+      st_o = qa > xo + 1 ? qa : xo + 1;
+      xo = en_o;
+    }
     int32_t st_n = 0;
     if (WRITE) st_n = qa > xn + 1 ? qa : xn + 1;
     uint32_t len;
-    if (qt == qrel) {
-      // touched quotient: lengths from the table; its counters are written later
-      len = sm.tnl[tp];
-      if (qo == qrel) {
-        occ &= occ - 1;
-        xo = (int32_t)sm.tst[tp] + (int32_t)sm.tol[tp] - 1;
-      }
-      if (WRITE) sm.tsn[tp] = (int16_t)st_n;
-      tp++;
+    if (touched & bit) {
+      // lengths from the table; the counters are written after the walk
+      const uint32_t t = tbase + (uint32_t)__popcll(touched & (bit - 1));
+      len = sm.tnl[t];
+      if (WRITE) sm.tsn[t] = (int16_t)st_n;
     } else {
-      occ &= occ - 1;
-      const int32_t st_o = qa > xo + 1 ? qa : xo + 1;
-      int bj = st_o >> 6;
-      if (bj >= nwin) return false;
-      if (bj != cw_blk) { cw_blk = bj; cw = sm.run_in[bj]; }
-      uint64_t w = cw >> (st_o & 63);
-      int32_t pos = st_o;
-      while (!w) {
-        bj++;
-        if (bj >= nwin) return false;
-        cw_blk = bj;
-        cw = sm.run_in[bj];
-        w = cw;
-        pos = 64 * bj;
-      }
-      const int32_t en_o = pos + __builtin_ctzll(w);
-      xo = en_o;
       len = (uint32_t)(en_o - st_o + 1);
-      if (WRITE)
-        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[raw_off(st_o + (int32_t)k)];
+      if (WRITE) {
+        if (len <= 4) {
+          // all reads issued before the writes: one LDS round trip per run
+          const uint8_t b0 = in[sin_off(st_o)];
+          const uint8_t b1 = len > 1 ? in[sin_off(st_o + 1)] : 0;
+          const uint8_t b2 = len > 2 ? in[sin_off(st_o + 2)] : 0;
+          const uint8_t b3 = len > 3 ? in[sin_off(st_o + 3)] : 0;
+          out[raw_off(st_n)] = b0;
+          if (len > 1) out[raw_off(st_n + 1)] = b1;
+          if (len > 2) out[raw_off(st_n + 2)] = b2;
+          if (len > 3) out[raw_off(st_n + 3)] = b3;
+        } else {
+          for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
+        }
+      }
     }
     if (len) {
-      newocc |= 1ULL << qrel;
+      newocc |= bit;
       compose(agg_a, agg_b, qa + (int32_t)len - 1, (int32_t)len);
       if (WRITE) {
         const int32_t en_n = st_n + (int32_t)len - 1;
@@ -472,7 +494,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       v[k] = g[c < nchunks ? c : nchunks - 1];
     }
 #pragma unroll
-    for (int k = 0; k < kChunkIters; k++) sm.raw_in[lane + 64 * k] = v[k];
+    for (int k = 0; k < kChunkIters; k++) sm.raw[lane + 64 * k] = v[k];
     if (src.cached) {
       for (uint32_t i = lane; i < m; i += 64) {
         sm.items[i] = (uint32_t)(items[s + i] - src.base);
@@ -486,16 +508,33 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
 
   // ---- 2. headers, old end-before per block, per-block item ranges ----
   uint64_t occ_w[kRounds];
+  uint32_t off_w[kRounds];
 #pragma unroll
   for (int k = 0; k < kRounds; k++) {
     const int j = lane + 64 * k;
     occ_w[k] = 0;
+    off_w[k] = 0;
     if (j < nwin) {
       uint32_t off;
       uint64_t run;
-      read_header((const uint8_t*)sm.raw_in, j, off, occ_w[k], run);
+      read_header((const uint8_t*)sm.raw, j, off, occ_w[k], run);
+      off_w[k] = off;
       sm.run_in[j] = run;
       sm.occ_in[j] = occ_w[k];
+      sm.tmask[j] = 0;
+      {
+        // slots of block j -> sin[16 j ..], realigned from the 81-byte stride
+        const uint8_t* raw = (const uint8_t*)sm.raw;
+        const uint32_t o = (uint32_t)j * GQF_BLOCK_BYTES + 17;
+        const uint32_t a = o & ~3u, sh = 8 * (o & 3u);
+        uint32_t prev = lds_dword(raw, a);
+#pragma unroll
+        for (int d = 0; d < 16; d++) {
+          const uint32_t next = lds_dword(raw, a + 4 * (d + 1));
+          sm.sin[17 * j + d] = sh ? (uint32_t)((((uint64_t)next << 32) | prev) >> sh) : prev;
+          prev = next;
+        }
+      }
       sm.xold[j] = off < GQF_OFFSET_SAT ? 64 * j - 1 + (int32_t)off
                                         : (int32_t)(true_x_global(qf, B0 + j) - win_base);
     }
@@ -515,6 +554,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
         sm.tq[t] = (uint16_t)(h >> 8);
         sm.tit0[t] = (uint16_t)i;
         atomicAdd(&sm.blk_cnt[h >> 14], 1u);
+        atomicOr((unsigned long long*)&sm.tmask[h >> 14], 1ULL << ((h >> 8) & 63));
       }
       carry += (uint32_t)__popcll(hm);
     }
@@ -572,7 +612,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
           it = sm.blk_start[j];
           ie = it + sm.blk_cnt[j];
         }
-        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, ie, occ_w[k], a, b, occ_new[k])
+        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], a, b, occ_new[k])
                              : walk_block<W, false>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occ_new[k]);
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
@@ -634,22 +674,22 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   const uint32_t wbytes = (uint32_t)nwrite * GQF_BLOCK_BYTES;
 
   // ---- 5. pass 2: assemble the new image in LDS ----
-  for (uint32_t c = lane; c < (wbytes + 15) / 16; c += 64) sm.raw_out[c] = make_uint4(0, 0, 0, 0);
+  for (uint32_t c = lane; c < (wbytes + 15) / 16; c += 64) sm.raw[c] = make_uint4(0, 0, 0, 0);
   for (int j = lane; j < nwrite; j += 64) sm.run_out[j] = 0;
   __syncthreads();
   {
-    const uint8_t* in = (const uint8_t*)sm.raw_in;
-    uint8_t* out = (uint8_t*)sm.raw_out;
+    const uint8_t* in = (const uint8_t*)sm.sin;
+    uint8_t* out = (uint8_t*)sm.raw;
     // the previous region's runs spilling into this window are kept
     for (int32_t i = lane; i <= sm.xold[0]; i += 64) {
-      out[raw_off(i)] = in[raw_off(i)];
+      out[raw_off(i)] = in[sin_off(i)];
       if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
         atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
     }
     // the meeting block keeps everything after the new end-before
     if (bstar < nwin) {
       for (int32_t i = sm.xnew[bstar] + 1 + lane; i < 64 * (bstar + 1); i += 64) {
-        out[raw_off(i)] = in[raw_off(i)];
+        out[raw_off(i)] = in[sin_off(i)];
         if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
           atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
       }
@@ -668,7 +708,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       }
       int32_t a, b;
       uint64_t occn;
-      if (fast) walk_block_table<W, true>(sm, j, nwin, it, ie, occ_w[k], a, b, occn);
+      if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], a, b, occn);
       else walk_block<W, true>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occn);
     }
   }
@@ -690,14 +730,14 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       uint32_t off;
       uint64_t occ;
       if (j == bstar) {
-        uint64_t run;
-        read_header((const uint8_t*)sm.raw_in, j, off, occ, run);
+        off = off_w[k];  // unchanged end-before: unchanged header
+        occ = occ_w[k];
       } else {
         const int32_t o = sm.xnew[j] - (64 * j - 1);
         off = o > GQF_OFFSET_SAT ? GQF_OFFSET_SAT : (uint32_t)o;
         occ = occ_new[k];
       }
-      write_header((uint8_t*)sm.raw_out, j, off, occ, sm.run_out[j]);
+      write_header((uint8_t*)sm.raw, j, off, occ, sm.run_out[j]);
     }
   }
   __syncthreads();
@@ -706,10 +746,10 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   {
     GQF_GLOBAL uint4* g = gqf_gptr_mut((uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
     const uint32_t full = wbytes / 16;
-    for (uint32_t c = lane; c < full; c += 64) g[c] = sm.raw_out[c];
+    for (uint32_t c = lane; c < full; c += 64) g[c] = sm.raw[c];
     const uint32_t tail = wbytes - full * 16;
     if ((uint32_t)lane < tail)
-      ((GQF_GLOBAL uint8_t*)g)[full * 16 + lane] = ((const uint8_t*)sm.raw_out)[full * 16 + lane];
+      ((GQF_GLOBAL uint8_t*)g)[full * 16 + lane] = ((const uint8_t*)sm.raw)[full * 16 + lane];
   }
   return kOk;
 }
