@@ -23,7 +23,6 @@ namespace {
 
 constexpr int kModeInsert = 0;
 constexpr int kModeDelete = 1;
-constexpr size_t kBlockPad = 4096;  // vector over-reads past the last block stay inside the allocation
 
 std::mutex g_mu;
 int g_last_error = 0;
@@ -157,6 +156,57 @@ uint64_t count_misses(QF* qf, uint64_t* vals, uint64_t n) {
   return (uint64_t)h;
 }
 
+inline uint64_t load_le64(const uint8_t* p) {
+  uint64_t v = 0;
+  for (int k = 7; k >= 0; k--) v = (v << 8) | p[k];
+  return v;
+}
+
+inline uint32_t select64_host(uint64_t w, uint32_t r) {  // position of the r-th set bit (0-based)
+  for (uint32_t k = 0; k < r; k++) w &= w - 1;
+  return (uint32_t)__builtin_ctzll(w);
+}
+
+// The reference's 81-byte image -> the device layout, with each block's
+// exact offset recomputed from the bit vectors: the runs of the quotients
+// below 64 b end at the P-th run end, P = occupied quotients below 64 b.
+// Rejects an image whose offset bytes disagree with that (min(offset, 255))
+// or whose run ends and occupied quotients do not pair up.
+// `out` receives the whole table allocation (gqf_table_bytes), slack zeroed.
+bool image_to_device_layout(const uint8_t* img, uint64_t nblocks, std::vector<uint8_t>& out) {
+  out.assign(gqf_table_bytes(nblocks), 0);
+  uint64_t occ_before = 0;   // occupied quotients < 64 b
+  uint64_t cb = 0;           // run-end cursor block
+  uint64_t ends_before = 0;  // run ends in blocks < cb
+  uint64_t total_ends = 0;
+  for (uint64_t b = 0; b < nblocks; b++) {
+    const uint8_t* src = img + b * GQF_REF_BLOCK_BYTES;
+    const uint64_t occ = load_le64(src + 1);
+    int64_t x = (int64_t)(b * 64) - 1;
+    if (occ_before > 0) {
+      while (cb < nblocks &&
+             ends_before + (uint64_t)__builtin_popcountll(load_le64(img + cb * GQF_REF_BLOCK_BYTES + 9)) < occ_before) {
+        ends_before += (uint64_t)__builtin_popcountll(load_le64(img + cb * GQF_REF_BLOCK_BYTES + 9));
+        cb++;
+      }
+      if (cb == nblocks) return false;
+      const uint64_t w = load_le64(img + cb * GQF_REF_BLOCK_BYTES + 9);
+      const int64_t end = (int64_t)(cb * 64 + select64_host(w, (uint32_t)(occ_before - ends_before - 1)));
+      if (end > x) x = end;
+    }
+    const uint64_t off = (uint64_t)(x - ((int64_t)(b * 64) - 1));
+    if (src[0] != (off < GQF_OFFSET_SAT ? off : GQF_OFFSET_SAT)) return false;
+    uint8_t* dst = out.data() + b * GQF_BLOCK_BYTES;
+    memcpy(dst, src + 1, 16);  // occupieds, runends (little endian both sides)
+    const uint32_t off32 = (uint32_t)off;
+    memcpy(dst + 16, &off32, 4);
+    memcpy(dst + GQF_SLOT_BASE, src + 17, 64);
+    occ_before += (uint64_t)__builtin_popcountll(occ);
+    total_ends += (uint64_t)__builtin_popcountll(load_le64(src + 9));
+  }
+  return total_ends == occ_before;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -182,9 +232,9 @@ extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
   h.range = h.nslots << h.key_remainder_bits;
   h.num_locks = h.xnslots / GQF_REF_LOCK_SLOTS + 2;
   h.nregions = (h.nblocks + GQF_REGION_BLOCKS - 1) / GQF_REGION_BLOCKS;
-  const size_t bytes = (size_t)h.nblocks * GQF_BLOCK_BYTES;
-  dev_alloc(&h.blocks, bytes + kBlockPad);
-  GQF_CHECK(hipMemset(h.blocks, 0, bytes + kBlockPad));  // qf_malloc memsets the image (src/gqf.cu:2081)
+  const size_t bytes = gqf_table_bytes(h.nblocks);
+  dev_alloc(&h.blocks, bytes);
+  GQF_CHECK(hipMemset(h.blocks, 0, bytes));  // qf_malloc memsets the image (src/gqf.cu:2081)
   dev_alloc(&h.status, 1);
   GQF_CHECK(hipMemset(h.status, 0, sizeof(uint32_t)));
   if (!bulk_config) {
@@ -304,7 +354,8 @@ extern "C" int qf_get_info(QF* qf, gqf_info* out) {
   out->range = h.range;
   out->num_locks = h.num_locks;
   out->nregions = h.nregions;
-  out->total_size_in_bytes = h.nblocks * GQF_BLOCK_BYTES;
+  out->total_size_in_bytes = h.nblocks * GQF_REF_BLOCK_BYTES;  // the reference's image size
+  out->device_bytes = gqf_table_bytes(h.nblocks);
   return 0;
 }
 
@@ -324,13 +375,23 @@ extern "C" void qf_clear_status(QF* qf) {
   GQF_CHECK(hipMemset(f->host.status, 0, sizeof(uint32_t)));
 }
 
+// The image these four calls exchange is the reference's packed 81-byte
+// block array (include/gqf_int.cuh:38-57), nblocks * 81 bytes: the device
+// layout is converted on the way out and rebuilt (exact offsets) on the way in.
+
 extern "C" int qf_export_image(QF* qf, void* host_dst, uint64_t nbytes) {
   Filter* f = lookup(qf);
   if (!f) return -1;
-  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  const uint64_t sz = f->host.nblocks * GQF_REF_BLOCK_BYTES;
   if (nbytes < sz) return -2;
   GQF_CHECK(hipDeviceSynchronize());
-  hipError_t e = hipMemcpy(host_dst, f->host.blocks, sz, hipMemcpyDeviceToHost);
+  uint8_t* tmp = nullptr;
+  hipError_t e = hipMalloc(&tmp, sz + 8);
+  if (e == hipSuccess) {
+    gqf_launch_export(f->dev, f->host.nblocks, tmp, nullptr);
+    e = hipMemcpy(host_dst, tmp, sz, hipMemcpyDeviceToHost);
+    GQF_CHECK(hipFree(tmp));
+  }
   note_error(e, "qf_export_image");
   return e == hipSuccess ? 0 : -3;
 }
@@ -338,32 +399,45 @@ extern "C" int qf_export_image(QF* qf, void* host_dst, uint64_t nbytes) {
 extern "C" int qf_import_image(QF* qf, const void* host_src, uint64_t nbytes) {
   Filter* f = lookup(qf);
   if (!f) return -1;
-  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  const uint64_t sz = f->host.nblocks * GQF_REF_BLOCK_BYTES;
   if (nbytes != sz) return -2;
+  std::vector<uint8_t> dev_img;
+  if (!image_to_device_layout((const uint8_t*)host_src, f->host.nblocks, dev_img)) {
+    fprintf(stderr, "libgqf: qf_import_image: inconsistent block image (offsets / run ends)\n");
+    return -4;
+  }
   GQF_CHECK(hipDeviceSynchronize());
-  hipError_t e = hipMemcpy(f->host.blocks, host_src, sz, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(f->host.blocks, dev_img.data(), dev_img.size(), hipMemcpyHostToDevice);
   note_error(e, "qf_import_image");
   return e == hipSuccess ? 0 : -3;
 }
 
+// stream-ordered: the conversion kernel runs on `stream`
 extern "C" int qf_export_image_device(QF* qf, void* dev_dst, uint64_t nbytes, void* stream) {
   Filter* f = lookup(qf);
   if (!f) return -1;
-  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  const uint64_t sz = f->host.nblocks * GQF_REF_BLOCK_BYTES;
   if (nbytes < sz) return -2;
-  hipError_t e = hipMemcpyAsync(dev_dst, f->host.blocks, sz, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  gqf_launch_export(f->dev, f->host.nblocks, (uint8_t*)dev_dst, (hipStream_t)stream);
+  hipError_t e = hipGetLastError();
   note_error(e, "qf_export_image_device");
   return e == hipSuccess ? 0 : -3;
 }
 
+// synchronises `stream`: the offsets are rebuilt on the host
 extern "C" int qf_import_image_device(QF* qf, const void* dev_src, uint64_t nbytes, void* stream) {
   Filter* f = lookup(qf);
   if (!f) return -1;
-  const uint64_t sz = f->host.nblocks * GQF_BLOCK_BYTES;
+  const uint64_t sz = f->host.nblocks * GQF_REF_BLOCK_BYTES;
   if (nbytes != sz) return -2;
-  hipError_t e = hipMemcpyAsync(f->host.blocks, dev_src, sz, hipMemcpyDeviceToDevice, (hipStream_t)stream);
-  note_error(e, "qf_import_image_device");
-  return e == hipSuccess ? 0 : -3;
+  std::vector<uint8_t> img(sz);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e == hipSuccess) e = hipMemcpy(img.data(), dev_src, sz, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    note_error(e, "qf_import_image_device");
+    return -3;
+  }
+  return qf_import_image(qf, img.data(), sz);
 }
 
 // Empties the table on the device (stream-ordered), as qf_reset empties the
@@ -371,7 +445,7 @@ extern "C" int qf_import_image_device(QF* qf, const void* dev_src, uint64_t nbyt
 extern "C" int qf_reset_device(QF* qf, void* stream) {
   Filter* f = lookup(qf);
   if (!f) return -1;
-  const size_t bytes = (size_t)f->host.nblocks * GQF_BLOCK_BYTES;
+  const size_t bytes = gqf_table_bytes(f->host.nblocks);
   hipError_t e = hipMemsetAsync(f->host.blocks, 0, bytes, (hipStream_t)stream);
   note_error(e, "qf_reset_device");
   return e == hipSuccess ? 0 : -3;

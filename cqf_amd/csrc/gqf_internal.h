@@ -1,25 +1,40 @@
 // gqf_internal.h -- device-side layout, constants and helpers shared by the
 // HIP kernels of libgqf (counting quotient filter for gfx950).
 //
-// The table is the reference's packed 81-byte block image, byte for byte
+// The reference stores the table as packed 81-byte blocks
 // (PASSIONLab/cqf include/gqf_int.cuh:38-57):
 //   byte 0       offset     u8   (saturates at 255 = "recompute")
 //   bytes 1..8   occupieds  u64 little endian, bit i = quotient 64b+i has a run
 //   bytes 9..16  runends    u64 little endian, bit i = slot 64b+i ends a run
 //   bytes 17..80 slots[64]  u8 remainders / counter digits
-// Keeping that image in HBM makes the device table the parity format itself.
+// On the device each block occupies one 128-byte line, every field aligned:
+//   bytes 0..7   occupieds   bytes 8..15 runends   bytes 16..19 offset (u32)
+//   bytes 20..63 zero        bytes 64..127 slots[64]
+// A probe loads the whole line in one round of back-to-back vector loads:
+// header and, for ~94% of runs at 90% load, the run itself.  Random access
+// on MI355X is bound by the number of dependent line requests, not bytes
+// (independent 16-byte and 128-byte random reads run at the same rate), so
+// one line per probe beats the 81-byte packing (1.4 lines, two dependent
+// rounds) although the table is 58% larger.
+// The device offset is exact -- end of the runs with quotient < 64 b, minus
+// (64 b - 1), floored at 0 -- never saturated, so no reader has to walk
+// back to recompute it; the export saturates it to the reference's byte.
+// The reference's 81-byte image is produced by qf_export_image and read by
+// qf_import_image; it is the parity format, byte for byte.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define GQF_BLOCK_BYTES 81
+#define GQF_REF_BLOCK_BYTES 81   // the reference's packed block (export format)
+#define GQF_BLOCK_BYTES 128      // device stride of one 64-slot block
+#define GQF_SLOT_BASE 64         // byte of slot 0 inside a device block
 #define GQF_SLOTS_PER_BLOCK 64
 #define GQF_REGION_BLOCKS 48                       // rebuild region; + 16 push blocks = one wave64
 #define GQF_REGION_SLOTS (GQF_REGION_BLOCKS * GQF_SLOTS_PER_BLOCK)
 #define GQF_REF_LOCK_SLOTS 8192ULL                 // reference point-lock region (src/gqf.cu:58)
 #define GQF_REF_LOCK_DIST 64                       // reference lock stride (src/gqf.cu:59)
-#define GQF_OFFSET_SAT 255
+#define GQF_OFFSET_SAT 255                         // the reference offset byte saturates here
 
 // status bits reported through qf_get_status()
 #define GQF_STATUS_NO_SPACE 0x1u          // a run would end past xnslots; region left unchanged
@@ -40,7 +55,7 @@ struct quotient_filter {
   uint64_t range;               // 2^(q+8)
   uint64_t num_locks;           // xnslots / 8192 + 2 (src/gqf.cu:1996)
   uint64_t nregions;            // ceil(nblocks / 64): even/odd rebuild regions
-  uint8_t* blocks;              // nblocks * 81 bytes (+ padding for vector over-reads)
+  uint8_t* blocks;              // nblocks * GQF_BLOCK_BYTES bytes (+ slack)
   uint32_t* locks;              // point-mode region locks (num_locks * 64 words), or null
   uint32_t* status;             // GQF_STATUS_* bits, sticky until cleared
 };
@@ -108,9 +123,7 @@ __device__ static inline int gqf_select64(uint64_t v, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// 81-byte block header access in global memory.  Blocks sit at 81-byte
-// stride, so the header is read as five aligned dwords and realigned with
-// byte funnel shifts.
+// block header access in global memory
 
 struct gqf_header {
   uint32_t offset;
@@ -118,29 +131,32 @@ struct gqf_header {
   uint64_t runends;
 };
 
-__device__ static inline uint32_t gqf_align_bytes(uint32_t hi, uint32_t lo, uint32_t s) {
-  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * s));
+__device__ static inline const GQF_GLOBAL uint4* gqf_block_ptr(const quotient_filter& qf, uint64_t b) {
+  return gqf_gptr((const uint4*)(qf.blocks + b * GQF_BLOCK_BYTES));
 }
 
-__device__ static inline gqf_header gqf_load_header(const uint8_t* blocks, uint64_t b) {
-  uint64_t addr = b * GQF_BLOCK_BYTES;
-  const GQF_GLOBAL uint32_t* p = gqf_gptr((const uint32_t*)(blocks + (addr & ~3ULL)));
-  uint32_t s = (uint32_t)(addr & 3ULL);
-  uint32_t d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3], d4 = p[4], d5 = p[5];
-  uint32_t a0 = gqf_align_bytes(d1, d0, s), a1 = gqf_align_bytes(d2, d1, s);
-  uint32_t a2 = gqf_align_bytes(d3, d2, s), a3 = gqf_align_bytes(d4, d3, s);
-  uint32_t a4 = gqf_align_bytes(d5, d4, s);
-  uint64_t lo = (uint64_t)a0 | ((uint64_t)a1 << 32);
-  uint64_t mid = (uint64_t)a2 | ((uint64_t)a3 << 32);
+__device__ static inline gqf_header gqf_load_header(const quotient_filter& qf, uint64_t b) {
+  const GQF_GLOBAL uint4* p = gqf_block_ptr(qf, b);
+  const uint4 v = p[0];
   gqf_header h;
-  h.offset = a0 & 0xFFu;
-  h.occupieds = (lo >> 8) | (mid << 56);
-  h.runends = (mid >> 8) | ((uint64_t)a4 << 56);
+  h.occupieds = (uint64_t)v.x | ((uint64_t)v.y << 32);
+  h.runends = (uint64_t)v.z | ((uint64_t)v.w << 32);
+  h.offset = ((const GQF_GLOBAL uint32_t*)p)[4];
   return h;
 }
 
-__device__ static inline uint64_t gqf_slot_addr(uint64_t slot) {
-  return (slot >> 6) * GQF_BLOCK_BYTES + 17 + (slot & 63);
+__device__ static inline uint64_t gqf_load_runends(const quotient_filter& qf, uint64_t b) {
+  return ((const GQF_GLOBAL uint64_t*)gqf_block_ptr(qf, b))[1];
+}
+
+// byte address of slot s inside the block array
+__host__ __device__ static inline uint64_t gqf_slot_addr(uint64_t s) {
+  return (s >> 6) * GQF_BLOCK_BYTES + GQF_SLOT_BASE + (s & 63);
+}
+
+// device table bytes: the blocks plus slack for vector over-reads
+__host__ __device__ static inline uint64_t gqf_table_bytes(uint64_t nblocks) {
+  return nblocks * GQF_BLOCK_BYTES + 4096;
 }
 
 // ---------------------------------------------------------------------------

@@ -29,6 +29,9 @@ void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uin
 void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsigned long long* misses,
                        hipStream_t s);
 
+// device layout -> the reference's packed 81-byte image (nblocks * 81 bytes)
+void gqf_launch_export(const QF* qf_dev, uint64_t nblocks, uint8_t* out, hipStream_t s);
+
 void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t skip, hipStream_t s);
 
 // Per-kernel event timing (off by default).  When enabled, the launchers

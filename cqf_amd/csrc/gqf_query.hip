@@ -6,72 +6,44 @@
 // counter whose remainder matches.
 //
 // Here the run bounds come straight from the home block's header: with
-// x = end of the runs of earlier blocks (offset byte), the k-th occupied
-// quotient of the block owns the k-th run end after x, so the run start and
-// end are two selects over the runends words -- no run_end recursion unless
-// the offset is saturated.
+// x = end of the runs of earlier blocks (the exact 32-bit offset the device
+// layout keeps), the k-th occupied quotient of the block owns the k-th run
+// end after x, so the run start and end are two selects over the runends
+// words -- no run_end recursion.
 #include "gqf_internal.h"
 #include "gqf_launch.h"
 
-namespace {
-
-__device__ int64_t next_runend_g(const QF& qf, uint64_t pos) {
-  uint64_t b = pos >> 6;
-  uint32_t bit = (uint32_t)(pos & 63);
-  while (b < qf.nblocks) {
-    uint64_t w = gqf_load_header(qf.blocks, b).runends >> bit;
-    if (w) return (int64_t)(b * 64 + bit + __builtin_ctzll(w));
-    b++;
-    bit = 0;
-  }
-  return -1;
-}
-
-// end of the runs with quotient < 64 b (at least 64 b - 1), for a block
-// whose stored offset is saturated (src/gqf.cu:619-630)
-__device__ int64_t true_x_g(const QF& qf, uint64_t b) {
-  uint64_t b0 = b;
-  gqf_header h = gqf_load_header(qf.blocks, b0);
-  while (b0 > 0 && h.offset == GQF_OFFSET_SAT) {
-    b0--;
-    h = gqf_load_header(qf.blocks, b0);
-  }
-  int64_t x = (int64_t)(b0 * 64) - 1 + (int64_t)h.offset;
-  for (uint64_t bb = b0; bb < b; bb++) {
-    uint64_t occ = gqf_load_header(qf.blocks, bb).occupieds;
-    while (occ) {
-      int64_t q = (int64_t)(bb * 64 + __builtin_ctzll(occ));
-      occ &= occ - 1;
-      int64_t st = q > x + 1 ? q : x + 1;
-      int64_t en = next_runend_g(qf, (uint64_t)st);
-      if (en < 0) return (int64_t)(qf.nblocks * 64);
-      x = en;
-    }
-  }
-  int64_t lo = (int64_t)(b * 64) - 1;
-  return x > lo ? x : lo;
-}
-
-}  // namespace
-
-// Count of `hash` (< range) in the table.
-__device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash) {
+// Count of `hash` (< range) in the table.  `row` is the calling thread's
+// 17-word LDS scratch row.
+//
+// One round of loads fetches the home block's whole line (header and the
+// 64 slots, six 16-byte loads to one 128-byte line).  When the quotient's
+// run lies inside the home block -- ~94% of occupied probes at 90% load --
+// the run is decoded from that copy (through the LDS row, since lanes index
+// it at different bytes); otherwise the run ends of later blocks and the
+// run's slots are read in a second round.
+__device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash, uint32_t* row) {
   const uint64_t q = hash >> 8;
   const uint64_t rem = hash & 0xFF;
   const uint64_t b = q >> 6;
   const uint32_t bit = (uint32_t)(q & 63);
-  const gqf_header hd = gqf_load_header(qf.blocks, b);
-  if (!((hd.occupieds >> bit) & 1ULL)) return 0;
+  const GQF_GLOBAL uint4* p = gqf_block_ptr(qf, b);
+  const uint4 h0 = p[0];
+  const uint4 h1 = p[1];
+  const uint4 s0 = p[4], s1 = p[5], s2 = p[6], s3 = p[7];
+  const uint64_t occ = (uint64_t)h0.x | ((uint64_t)h0.y << 32);
+  const uint64_t runends = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+  if (!((occ >> bit) & 1ULL)) return 0;
 
-  const int64_t xb = hd.offset < GQF_OFFSET_SAT ? (int64_t)(b * 64) - 1 + (int64_t)hd.offset
-                                                : true_x_g(qf, b);
-  const int rank = __popcll(hd.occupieds & ((2ULL << bit) - 1ULL));  // >= 1
+  // device offsets are exact: the end of the runs with quotient < 64 b
+  const int64_t xb = (int64_t)(b * 64) - 1 + (int64_t)h1.x;
+  const int rank = __popcll(occ & ((2ULL << bit) - 1ULL));  // >= 1
 
   // the (rank-2)'th and (rank-1)'th run ends at or after xb + 1
   int64_t prev_end = xb, end = -1;
   uint64_t pos = (uint64_t)(xb + 1);
   uint64_t bb = pos >> 6;
-  uint64_t word = (bb == b ? hd.runends : gqf_load_header(qf.blocks, bb).runends) & (~0ULL << (pos & 63));
+  uint64_t word = (bb == b ? runends : gqf_load_runends(qf, bb)) & (~0ULL << (pos & 63));
   int seen = 0;
   for (;;) {
     int c = __popcll(word);
@@ -84,14 +56,30 @@ __device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash) {
     seen += c;
     bb++;
     if (bb >= qf.nblocks) return 0;
-    word = gqf_load_header(qf.blocks, bb).runends;
+    word = gqf_load_runends(qf, bb);
   }
   int64_t start = prev_end + 1;
   if (start < (int64_t)q) start = (int64_t)q;
   const uint32_t len = (uint32_t)(end - start + 1);
+  uint32_t idx = 0;
+  if ((uint64_t)end >> 6 == b) {
+    // run inside the home block: decode from the loaded line
+    row[0] = s0.x; row[1] = s0.y; row[2] = s0.z; row[3] = s0.w;
+    row[4] = s1.x; row[5] = s1.y; row[6] = s1.z; row[7] = s1.w;
+    row[8] = s2.x; row[9] = s2.y; row[10] = s2.z; row[11] = s2.w;
+    row[12] = s3.x; row[13] = s3.y; row[14] = s3.z; row[15] = s3.w;
+    const uint8_t* bytes = (const uint8_t*)row + (start & 63);
+    auto slot = [&](uint32_t k) -> uint64_t { return bytes[k]; };
+    do {
+      uint64_t r, c;
+      uint32_t e = gqf_decode_counter(slot, len, idx, &r, &c);
+      if (r == rem) return c;
+      idx = e + 1;
+    } while (idx < len);
+    return 0;
+  }
   const GQF_GLOBAL uint8_t* blocks = gqf_gptr(qf.blocks);
   auto slot = [&](uint32_t k) -> uint64_t { return blocks[gqf_slot_addr((uint64_t)start + k)]; };
-  uint32_t idx = 0;
   do {
     uint64_t r, c;
     uint32_t e = gqf_decode_counter(slot, len, idx, &r, &c);
@@ -104,11 +92,13 @@ __device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash) {
 extern "C" __global__ void __launch_bounds__(256)
 gqf_bulk_get_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ vals, uint64_t n,
                     uint64_t* __restrict__ returns) {
+  __shared__ uint32_t rows[256 * 17];
   const QF qf = *qfp;
   const uint64_t mask = qf.range - 1;
+  uint32_t* row = rows + 17 * threadIdx.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    returns[i] = gqf_count_hash(qf, gqf_hash64(vals[i], mask));
+    returns[i] = gqf_count_hash(qf, gqf_hash64(vals[i], mask), row);
   }
 }
 
@@ -116,12 +106,14 @@ gqf_bulk_get_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ val
 extern "C" __global__ void __launch_bounds__(256)
 gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ vals, uint64_t n,
                        unsigned long long* __restrict__ misses) {
+  __shared__ uint32_t rows[256 * 17];
   const QF qf = *qfp;
   const uint64_t mask = qf.range - 1;
+  uint32_t* row = rows + 17 * threadIdx.x;
   uint32_t mine = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    mine += gqf_count_hash(qf, gqf_hash64(vals[i], mask)) < 1;
+    mine += gqf_count_hash(qf, gqf_hash64(vals[i], mask), row) < 1;
   }
   // wave reduction, one atomic per wave
   for (int d = 32; d >= 1; d >>= 1) mine += __shfl_down(mine, d, 64);

@@ -44,11 +44,8 @@ enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 template <int W>
 struct Smem {
   // whole 64-chunk rows so staging can load and store branch-free
-  // the raw 81-byte image: staged in here, then (once slots and headers are
-  // extracted) reused to assemble the output; whole 64-chunk rows so staging
-  // can load and store branch-free
-  uint4 raw[((W * GQF_BLOCK_BYTES + 15) / 16 + 63) / 64 * 64];
-  uint32_t sin[W * 17];  // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
+  uint32_t sin[W * 17];   // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
+  uint32_t sout[W * 17];  // new slots, same rows
   uint64_t run_in[W];
   uint64_t run_out[W];
   uint64_t tmask[W];     // quotients of block j touched by the items (fast path)
@@ -75,58 +72,10 @@ struct Smem {
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 
-// byte offset of window-relative slot s in the raw 81-byte image
-__device__ inline uint32_t raw_off(int32_t s) { return (uint32_t)(s >> 6) * GQF_BLOCK_BYTES + 17 + (s & 63); }
-
-// byte offset of window-relative slot p in sin: one 68-byte row per block,
-// so lanes reading "their" block at the same column hit different banks
+// byte offset of window-relative slot p in the slot rows (sin / sout): one
+// 68-byte row per block, so lanes reading "their" block at the same column
+// hit different LDS banks
 __device__ inline uint32_t sin_off(int32_t p) { return (uint32_t)(p >> 6) * 68 + (p & 63); }
-
-// LDS dword read at a 4-byte aligned byte offset of a byte buffer
-__device__ inline uint32_t lds_dword(const uint8_t* base, uint32_t byte_off) {
-  return *(const uint32_t*)(base + byte_off);
-}
-
-// ---------------------------------------------------------------------------
-// global-image helpers (used when a stored offset is saturated)
-
-__device__ int64_t next_runend_global(const QF& qf, uint64_t pos) {
-  uint64_t b = pos >> 6;
-  uint32_t bit = (uint32_t)(pos & 63);
-  while (b < qf.nblocks) {
-    uint64_t w = gqf_load_header(qf.blocks, b).runends >> bit;
-    if (w) return (int64_t)(b * 64 + bit + __builtin_ctzll(w));
-    b++;
-    bit = 0;
-  }
-  return -1;
-}
-
-// End of the runs with quotient < 64*b, at least 64*b - 1: the quantity the
-// block offset encodes (src/gqf.cu:619-630).  Walks back to a block with an
-// unsaturated offset, then forward over the runs in between.
-__device__ int64_t true_x_global(const QF& qf, uint64_t b) {
-  uint64_t b0 = b;
-  gqf_header h = gqf_load_header(qf.blocks, b0);
-  while (b0 > 0 && h.offset == GQF_OFFSET_SAT) {
-    b0--;
-    h = gqf_load_header(qf.blocks, b0);
-  }
-  int64_t x = (int64_t)(b0 * 64) - 1 + (int64_t)h.offset;
-  for (uint64_t bb = b0; bb < b; bb++) {
-    uint64_t occ = gqf_load_header(qf.blocks, bb).occupieds;
-    while (occ) {
-      int64_t q = (int64_t)(bb * 64 + __builtin_ctzll(occ));
-      occ &= occ - 1;
-      int64_t st = q > x + 1 ? q : x + 1;
-      int64_t en = next_runend_global(qf, (uint64_t)st);
-      if (en < 0) return (int64_t)(qf.nblocks * 64);
-      x = en;
-    }
-  }
-  int64_t lo = (int64_t)(b * 64) - 1;
-  return x > lo ? x : lo;
-}
 
 __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   // (a, b) then (a2, b2): x -> max(a2, max(a, x + b) + b2)
@@ -189,11 +138,11 @@ template <int W, bool WRITE>
 __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
                                                    int32_t st_o, uint32_t old_len, int32_t st_n, int mode) {
   const uint8_t* in = (const uint8_t*)sm.sin;
-  uint8_t* out = (uint8_t*)sm.raw;
+  uint8_t* out = (uint8_t*)sm.sout;
   uint32_t len = 0;
   auto old_slot = [&](uint32_t k) -> uint64_t { return in[sin_off(st_o + (int32_t)k)]; };
   auto put = [&](uint32_t k, uint64_t v) {
-    if (WRITE) out[raw_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
+    if (WRITE) out[sin_off(st_n + (int32_t)(len + k))] = (uint8_t)v;
   };
   uint32_t oi = 0, o_end = 0;
   uint64_t o_rem = 0, o_cnt = 0;
@@ -223,7 +172,7 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
     if (have_old && (!have_new || o_rem < n_rem)) {
       const uint32_t clen = o_end - oi + 1;  // unchanged counter: copy its bytes
       if (WRITE)
-        for (uint32_t k = 0; k < clen; k++) out[raw_off(st_n + (int32_t)(len + k))] = in[sin_off(st_o + (int32_t)(oi + k))];
+        for (uint32_t k = 0; k < clen; k++) out[sin_off(st_n + (int32_t)(len + k))] = in[sin_off(st_o + (int32_t)(oi + k))];
       len += clen;
       next_old();
     } else if (!have_old || n_rem < o_rem) {
@@ -248,7 +197,7 @@ template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const Items& src, uint32_t it, uint32_t ie,
                            int mode, uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
   const uint8_t* in = (const uint8_t*)sm.sin;
-  uint8_t* out = (uint8_t*)sm.raw;
+  uint8_t* out = (uint8_t*)sm.sout;
   const int32_t blk_base = 64 * j;
   int32_t xo = sm.xold[j];
   int32_t xn = WRITE ? sm.xnew[j] : 0;
@@ -297,7 +246,7 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       // untouched run: same bytes at the new position
       len = old_len;
       if (WRITE)
-        for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
+        for (uint32_t k = 0; k < len; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
     } else {
       const uint32_t it0 = it;
       while (it < ie && (int32_t)(src.hash(sm.items, it) >> 8) == qa) it++;
@@ -365,7 +314,7 @@ template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tbase, uint64_t occ,
                                                  int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
   const uint8_t* in = (const uint8_t*)sm.sin;
-  uint8_t* out = (uint8_t*)sm.raw;
+  uint8_t* out = (uint8_t*)sm.sout;
   const uint64_t touched = sm.tmask[j];
   const int32_t blk_base = 64 * j;
   int32_t xo = sm.xold[j];
@@ -415,12 +364,12 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
           const uint8_t b1 = len > 1 ? in[sin_off(st_o + 1)] : 0;
           const uint8_t b2 = len > 2 ? in[sin_off(st_o + 2)] : 0;
           const uint8_t b3 = len > 3 ? in[sin_off(st_o + 3)] : 0;
-          out[raw_off(st_n)] = b0;
-          if (len > 1) out[raw_off(st_n + 1)] = b1;
-          if (len > 2) out[raw_off(st_n + 2)] = b2;
-          if (len > 3) out[raw_off(st_n + 3)] = b3;
+          out[sin_off(st_n)] = b0;
+          if (len > 1) out[sin_off(st_n + 1)] = b1;
+          if (len > 2) out[sin_off(st_n + 2)] = b2;
+          if (len > 3) out[sin_off(st_n + 3)] = b3;
         } else {
-          for (uint32_t k = 0; k < len; k++) out[raw_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
+          for (uint32_t k = 0; k < len; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
         }
       }
     }
@@ -437,30 +386,6 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
   return true;
 }
 
-// header of window block j from the raw LDS image
-__device__ inline void read_header(const uint8_t* raw, int j, uint32_t& off, uint64_t& occ, uint64_t& run) {
-  const uint8_t* p = raw + j * GQF_BLOCK_BYTES;
-  uint64_t o = 0, r = 0;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    o |= (uint64_t)p[1 + k] << (8 * k);
-    r |= (uint64_t)p[9 + k] << (8 * k);
-  }
-  off = p[0];
-  occ = o;
-  run = r;
-}
-
-__device__ inline void write_header(uint8_t* raw, int j, uint32_t off, uint64_t occ, uint64_t run) {
-  uint8_t* p = raw + j * GQF_BLOCK_BYTES;
-  p[0] = (uint8_t)off;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    p[1 + k] = (uint8_t)(occ >> (8 * k));
-    p[9 + k] = (uint8_t)(run >> (8 * k));
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Region rebuild.  All 64 lanes of the (single-wave) workgroup call it.
 // Lane l handles window blocks l, l + 64, ...
@@ -470,8 +395,6 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
                               Smem<W>& sm) {
   constexpr int kRounds = W / 64;
-  constexpr int kChunks = (W * GQF_BLOCK_BYTES + 15) / 16;
-  constexpr int kChunkIters = (kChunks + 63) / 64;
   const int lane = lane_id();
   const uint64_t B0 = r * R;
   const int nwin = (int)min((uint64_t)W, qf.nblocks - B0);
@@ -482,19 +405,25 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   const Items src{items, counts, s, B0 << 14, m <= (uint32_t)kItemCap};
 
   // ---- 1. stage the window and the items ----
+  // lane j reads block j's line: header into registers, slots into LDS row j
+  uint64_t occ_w[kRounds];
+  uint32_t off_w[kRounds];
   {
-    // 16-byte aligned (3888 r); every lane issues all its loads (clamped
-    // in bounds) before the first LDS write, so they are in flight together
-    const GQF_GLOBAL uint4* g = gqf_gptr((const uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
-    const uint32_t nchunks = ((uint32_t)nwin * GQF_BLOCK_BYTES + 15) / 16;
-    uint4 v[kChunkIters];
+    const GQF_GLOBAL uint4* g = gqf_block_ptr(qf, B0);
+    uint4 hv[kRounds], s0[kRounds], s1[kRounds], s2[kRounds], s3[kRounds];
+    uint32_t ov[kRounds];
 #pragma unroll
-    for (int k = 0; k < kChunkIters; k++) {
-      const uint32_t c = (uint32_t)lane + 64u * k;
-      v[k] = g[c < nchunks ? c : nchunks - 1];
+    for (int k = 0; k < kRounds; k++) {
+      const int j = lane + 64 * k;
+      const int jj = j < nwin ? j : nwin - 1;  // clamped: every lane loads, all in flight together
+      const GQF_GLOBAL uint4* b = g + (size_t)jj * (GQF_BLOCK_BYTES / 16);
+      hv[k] = b[0];
+      ov[k] = ((const GQF_GLOBAL uint32_t*)b)[4];
+      s0[k] = b[4];
+      s1[k] = b[5];
+      s2[k] = b[6];
+      s3[k] = b[7];
     }
-#pragma unroll
-    for (int k = 0; k < kChunkIters; k++) sm.raw[lane + 64 * k] = v[k];
     if (src.cached) {
       for (uint32_t i = lane; i < m; i += 64) {
         sm.items[i] = (uint32_t)(items[s + i] - src.base);
@@ -503,42 +432,31 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     }
     sm.blk_cnt[lane] = 0;
     if (lane == 0) { sm.first_invalid = nwin; sm.bstar = -1; sm.result = kOk; }
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      const int j = lane + 64 * k;
+      occ_w[k] = 0;
+      off_w[k] = 0;
+      if (j < nwin) {
+        const uint64_t occ = (uint64_t)hv[k].x | ((uint64_t)hv[k].y << 32);
+        const uint64_t run = (uint64_t)hv[k].z | ((uint64_t)hv[k].w << 32);
+        occ_w[k] = occ;
+        off_w[k] = ov[k];
+        sm.run_in[j] = run;
+        sm.occ_in[j] = occ;
+        sm.tmask[j] = 0;
+        uint32_t* row = sm.sin + 17 * j;
+        row[0] = s0[k].x; row[1] = s0[k].y; row[2] = s0[k].z; row[3] = s0[k].w;
+        row[4] = s1[k].x; row[5] = s1[k].y; row[6] = s1[k].z; row[7] = s1[k].w;
+        row[8] = s2[k].x; row[9] = s2[k].y; row[10] = s2[k].z; row[11] = s2[k].w;
+        row[12] = s3[k].x; row[13] = s3[k].y; row[14] = s3[k].z; row[15] = s3[k].w;
+        sm.xold[j] = 64 * j - 1 + (int32_t)ov[k];  // device offsets are exact (never saturated)
+      }
+    }
   }
   __syncthreads();
 
-  // ---- 2. headers, old end-before per block, per-block item ranges ----
-  uint64_t occ_w[kRounds];
-  uint32_t off_w[kRounds];
-#pragma unroll
-  for (int k = 0; k < kRounds; k++) {
-    const int j = lane + 64 * k;
-    occ_w[k] = 0;
-    off_w[k] = 0;
-    if (j < nwin) {
-      uint32_t off;
-      uint64_t run;
-      read_header((const uint8_t*)sm.raw, j, off, occ_w[k], run);
-      off_w[k] = off;
-      sm.run_in[j] = run;
-      sm.occ_in[j] = occ_w[k];
-      sm.tmask[j] = 0;
-      {
-        // slots of block j -> sin[16 j ..], realigned from the 81-byte stride
-        const uint8_t* raw = (const uint8_t*)sm.raw;
-        const uint32_t o = (uint32_t)j * GQF_BLOCK_BYTES + 17;
-        const uint32_t a = o & ~3u, sh = 8 * (o & 3u);
-        uint32_t prev = lds_dword(raw, a);
-#pragma unroll
-        for (int d = 0; d < 16; d++) {
-          const uint32_t next = lds_dword(raw, a + 4 * (d + 1));
-          sm.sin[17 * j + d] = sh ? (uint32_t)((((uint64_t)next << 32) | prev) >> sh) : prev;
-          prev = next;
-        }
-      }
-      sm.xold[j] = off < GQF_OFFSET_SAT ? 64 * j - 1 + (int32_t)off
-                                        : (int32_t)(true_x_global(qf, B0 + j) - win_base);
-    }
-  }
+  // ---- 2. per-block item ranges / quotient table ----
   const bool fast = src.cached;
   if (fast) {
     // list the distinct quotients of the items (sorted, so heads compact in order)
@@ -671,25 +589,24 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   if (res != kOk) return res;
   const int bstar = sm.bstar;  // blocks [0, bstar) re-laid out, bstar keeps its own runs
   const int nwrite = bstar < nwin ? bstar + 1 : nwin;
-  const uint32_t wbytes = (uint32_t)nwrite * GQF_BLOCK_BYTES;
 
   // ---- 5. pass 2: assemble the new image in LDS ----
-  for (uint32_t c = lane; c < (wbytes + 15) / 16; c += 64) sm.raw[c] = make_uint4(0, 0, 0, 0);
+  for (uint32_t c = lane; c < (uint32_t)nwrite * 17; c += 64) sm.sout[c] = 0;
   for (int j = lane; j < nwrite; j += 64) sm.run_out[j] = 0;
   __syncthreads();
   {
     const uint8_t* in = (const uint8_t*)sm.sin;
-    uint8_t* out = (uint8_t*)sm.raw;
+    uint8_t* out = (uint8_t*)sm.sout;
     // the previous region's runs spilling into this window are kept
     for (int32_t i = lane; i <= sm.xold[0]; i += 64) {
-      out[raw_off(i)] = in[sin_off(i)];
+      out[sin_off(i)] = in[sin_off(i)];
       if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
         atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
     }
     // the meeting block keeps everything after the new end-before
     if (bstar < nwin) {
       for (int32_t i = sm.xnew[bstar] + 1 + lane; i < 64 * (bstar + 1); i += 64) {
-        out[raw_off(i)] = in[sin_off(i)];
+        out[sin_off(i)] = in[sin_off(i)];
         if ((sm.run_in[i >> 6] >> (i & 63)) & 1ULL)
           atomicOr((unsigned long long*)&sm.run_out[i >> 6], 1ULL << (i & 63));
       }
@@ -723,33 +640,34 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kRounds; k++) {
-    const int j = lane + 64 * k;
-    if (j < nwrite) {
-      uint32_t off;
-      uint64_t occ;
-      if (j == bstar) {
-        off = off_w[k];  // unchanged end-before: unchanged header
-        occ = occ_w[k];
-      } else {
-        const int32_t o = sm.xnew[j] - (64 * j - 1);
-        off = o > GQF_OFFSET_SAT ? GQF_OFFSET_SAT : (uint32_t)o;
-        occ = occ_new[k];
-      }
-      write_header((uint8_t*)sm.raw, j, off, occ, sm.run_out[j]);
-    }
-  }
-  __syncthreads();
-
-  // ---- 6. store blocks [0, nwrite) ----
+  // ---- 6. store blocks [0, nwrite): lane j writes block j's line ----
   {
     GQF_GLOBAL uint4* g = gqf_gptr_mut((uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
-    const uint32_t full = wbytes / 16;
-    for (uint32_t c = lane; c < full; c += 64) g[c] = sm.raw[c];
-    const uint32_t tail = wbytes - full * 16;
-    if ((uint32_t)lane < tail)
-      ((GQF_GLOBAL uint8_t*)g)[full * 16 + lane] = ((const uint8_t*)sm.raw)[full * 16 + lane];
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      const int j = lane + 64 * k;
+      if (j < nwrite) {
+        uint32_t off;
+        uint64_t occ;
+        if (j == bstar) {
+          off = off_w[k];  // unchanged end-before: unchanged header
+          occ = occ_w[k];
+        } else {
+          const int32_t o = sm.xnew[j] - (64 * j - 1);
+          off = (uint32_t)o;
+          occ = occ_new[k];
+        }
+        const uint64_t run = sm.run_out[j];
+        const uint32_t* row = sm.sout + 17 * j;
+        GQF_GLOBAL uint4* b = g + (size_t)j * (GQF_BLOCK_BYTES / 16);
+        b[0] = make_uint4((uint32_t)occ, (uint32_t)(occ >> 32), (uint32_t)run, (uint32_t)(run >> 32));
+        b[1] = make_uint4(off, 0, 0, 0);
+        b[4] = make_uint4(row[0], row[1], row[2], row[3]);
+        b[5] = make_uint4(row[4], row[5], row[6], row[7]);
+        b[6] = make_uint4(row[8], row[9], row[10], row[11]);
+        b[7] = make_uint4(row[12], row[13], row[14], row[15]);
+      }
+    }
   }
   return kOk;
 }

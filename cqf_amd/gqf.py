@@ -42,7 +42,7 @@ EXPORTS = [
 class GqfInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "nslots", "xnslots", "nblocks", "key_bits", "key_remainder_bits", "bits_per_slot",
-        "range", "num_locks", "nregions", "total_size_in_bytes")]
+        "range", "num_locks", "nregions", "total_size_in_bytes", "device_bytes")]
 
 
 _lib = None
@@ -204,6 +204,21 @@ class GQF:
         rc = self._lib.qf_import_image(self.handle, img.ctypes.data_as(ctypes.c_void_p), img.size)
         if rc != 0:
             raise RuntimeError(f"qf_import_image rc={rc}")
+
+    def export_image_device(self, out=None, stream=None):
+        """The 81-byte-block image into a uint8 device tensor (stream-ordered)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        rc = self._lib.qf_export_image_device(self.handle, _ptr(out), out.numel(), _stream_ptr(stream))
+        if rc != 0:
+            raise RuntimeError(f"qf_export_image_device rc={rc}")
+        return out
+
+    def import_image_device(self, img, stream=None):
+        rc = self._lib.qf_import_image_device(self.handle, _ptr(img), img.numel(), _stream_ptr(stream))
+        if rc != 0:
+            raise RuntimeError(f"qf_import_image_device rc={rc}")
 
 
 def fill_splitmix64(out, seed, skip=0, stream=None):

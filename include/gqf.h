@@ -87,7 +87,9 @@ void point_bulk_insert_wrapper(QF *qf, uint64_t *hashes, uint64_t nitems);
 
 typedef struct gqf_info {
   uint64_t nslots, xnslots, nblocks, key_bits, key_remainder_bits, bits_per_slot;
-  uint64_t range, num_locks, nregions, total_size_in_bytes;
+  uint64_t range, num_locks, nregions;
+  uint64_t total_size_in_bytes;  /* bytes of the 81-byte-block image (as the reference's) */
+  uint64_t device_bytes;         /* bytes of the device table (84 per block + alignment) */
 } gqf_info;
 
 int qf_get_info(QF *qf, gqf_info *out);
@@ -97,7 +99,8 @@ void qf_clear_status(QF *qf);
 /* copy the 81-byte block image (total_size_in_bytes) to / from host memory */
 int qf_export_image(QF *qf, void *host_dst, uint64_t nbytes);
 int qf_import_image(QF *qf, const void *host_src, uint64_t nbytes);
-/* copy the block image to / from device memory (stream-ordered) */
+/* the same image in device memory: export is stream-ordered, import
+ * synchronises the stream (offsets are rebuilt on the host) */
 int qf_export_image_device(QF *qf, void *dev_dst, uint64_t nbytes, void *stream);
 int qf_import_image_device(QF *qf, const void *dev_src, uint64_t nbytes, void *stream);
 /* empty the table (stream-ordered); mirrors qf_reset, src/gqf.cu:2283-2299 */
