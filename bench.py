@@ -6,9 +6,11 @@ filled to 90% load with bulk_insert in batches of 2^24 keys
 (splitmix64 seed 3, 241,591,910 keys), then bulk_get runs over the inserted
 keys and over 2^28 disjoint keys (outputs N .. N+2^28-1 of the same stream,
 distinct from the inserted ones because splitmix64 is a bijection of its
-state).  Keys are generated on the GPU and resident in HBM before the timed
-calls start; each step re-copies them (bulk_insert overwrites its keys with
-the sorted hashes) and empties the table.
+state), then bulk_delete removes the half of the inserted keys whose
+splitmix64(seed 7) selector output is odd, in batches of 2^24.  Keys are
+generated on the GPU and resident in HBM before the timed calls start; each
+step re-copies them (bulk mutations overwrite their keys with the sorted
+hashes) and empties the table.
 
 Prints one JSON line.  `value` is (inserts + queries) / (summed time of the
 bulk calls), in Mops/s; the separate insert and query rates, p50/p99 batch
@@ -39,11 +41,28 @@ def parse():
     ap.add_argument("--batch-log2", type=int, default=24)
     ap.add_argument("--load", type=float, default=0.9)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-nbits", type=int, default=20)
+    ap.add_argument("--cpu-nbits", type=int, default=26,
+                    help="table size of the CPU baseline sample (2^26: about 25 s on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_bulk_get_q28.md"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_q28.md"),
                     help="rocprofv3 counter summary holding the measured HBM bytes per launch")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB -> 1 GiB elementwise kernel 5 times (counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE: 5 x
+    bitwise_not over 2^27 int64 (reads 1 GiB, writes 1 GiB per launch)."""
+    import torch
+    x = torch.arange(1 << 27, dtype=torch.int64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.bitwise_not(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "bitwise_not int64", "read_bytes": 1 << 30, "write_bytes": 1 << 30,
+                      "launches": 5}), flush=True)
 
 
 def read_traffic(path, kernel):
@@ -60,15 +79,17 @@ def read_traffic(path, kernel):
     return None
 
 
-def cpu_baseline(nbits):
-    """Oracle (reference host path restated in C) on the reference's CPU
-    configuration: q = nbits, 90% load, splitmix64 seed 1 inserts, then the
-    same keys and 2^nbits disjoint seed-2 keys queried; one core."""
+def cpu_baseline(nbits, seed):
+    """The oracle (the reference host path, qf_insert / qf_count_key_value,
+    restated in C) on a bounded sample of the same workload: a 2^nbits-slot
+    table filled to 90% load with splitmix64(seed) keys one qf_insert at a
+    time, then the inserted keys and 2^nbits disjoint keys (the next outputs
+    of the same stream) queried; one core."""
     from oracle import oracle as O
-    n = int(0.9 * (1 << nbits))
-    keys = O.splitmix64(1, n)
-    q = O.splitmix64(2, 1 << nbits)
     import numpy as np
+    n = int(0.9 * (1 << nbits))
+    keys = O.splitmix64(seed, n)
+    q = O.splitmix64(seed, 1 << nbits, skip=n)
     qk = np.concatenate([keys, q])
     t_ins, t_q, hits = O.time_insert_query(nbits, keys, qk)
     ops = n + qk.size
@@ -77,8 +98,9 @@ def cpu_baseline(nbits):
         "unit": "Mops/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"q={nbits} table, {n} inserts to 90% load then {qk.size} queries "
-                   f"(reference host path restated in C, single thread)"),
+        "sample": (f"the same workload on a 2^{nbits}-slot table: {n} sequential inserts of "
+                   f"splitmix64(seed={seed}) keys to 90% load, then {qk.size} queries (inserted + "
+                   f"2^{nbits} disjoint); reference host path restated in C, single thread"),
         "inserts_mops": round(n / t_ins / 1e6, 3),
         "queries_mops": round(qk.size / t_q / 1e6, 3),
         "seconds": round(t_ins + t_q, 3),
@@ -93,6 +115,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(filters shard by hash prefix or by filter; see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import numpy as np
     import torch
@@ -115,21 +141,34 @@ def main():
     ret_dis = torch.empty_like(disjoint)
     gqf.fill_splitmix64(master, args.seed, 0, stream)
     gqf.fill_splitmix64(disjoint, args.seed, n_ins, stream)
+    # the deleted half: key i iff output i of splitmix64(7) is odd
+    sel = torch.empty_like(master)
+    gqf.fill_splitmix64(sel, 7, 0, stream)
+    half = master[(sel & 1) == 1].contiguous()
+    kept = master[(sel & 1) == 0].contiguous()
+    del sel
+    dwork = torch.empty_like(half)
     torch.cuda.synchronize()
     bounds = list(range(0, n_ins, batch)) + [n_ins]
+    dbounds = list(range(0, half.numel(), batch)) + [half.numel()]
 
     def step(record):
         f.reset(stream)
         work.copy_(master)
-        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(bounds) + 2)]
+        dwork.copy_(half)
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(bounds) + len(dbounds) + 1)]
         evs[0].record(stream)
         for i in range(len(bounds) - 1):
             f.bulk_insert(work[bounds[i]:bounds[i + 1]], 0, stream)
             evs[i + 1].record(stream)
+        nb = len(bounds) - 1
         f.bulk_get(master, ret_ins, stream)
-        evs[-2].record(stream)
+        evs[nb + 1].record(stream)
         f.bulk_get(disjoint, ret_dis, stream)
-        evs[-1].record(stream)
+        evs[nb + 2].record(stream)
+        for i in range(len(dbounds) - 1):
+            f.bulk_delete(dwork[dbounds[i]:dbounds[i + 1]], 0, stream)
+            evs[nb + 3 + i].record(stream)
         if record is not None:
             record.append(evs)
 
@@ -146,8 +185,9 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    ins_ms, q_ms, batch_lat, query_lat = 0.0, 0.0, [], []
+    ins_ms, q_ms, del_ms, batch_lat, query_lat, del_lat = 0.0, 0.0, 0.0, [], [], []
     nb = len(bounds) - 1
+    nd = len(dbounds) - 1
     for evs in recs:
         for i in range(nb):
             batch_lat.append(evs[i].elapsed_time(evs[i + 1]))
@@ -155,6 +195,9 @@ def main():
         query_lat.append(evs[nb].elapsed_time(evs[nb + 1]))
         query_lat.append(evs[nb + 1].elapsed_time(evs[nb + 2]))
         q_ms += evs[nb].elapsed_time(evs[nb + 2])
+        for i in range(nd):
+            del_lat.append(evs[nb + 2 + i].elapsed_time(evs[nb + 3 + i]))
+        del_ms += evs[nb + 2].elapsed_time(evs[nb + 2 + nd])
 
     # kernel-level times from the library's own events
     kern = {k: gqf.profile_read(k) for k in ("sort_pass", "region_bounds", "rebuild_phase",
@@ -166,13 +209,9 @@ def main():
     present_ok = bool((ret_ins > 0).all().item())
     fp = int((ret_dis > 0).sum().item())
     status = f.status()
+    # after the half delete, every kept key is still present
+    kept_ok = bool((f.bulk_get(kept) > 0).all().item()) if kept.numel() else True
 
-    # exact keys per rebuild launch: work holds each batch's sorted hashes
-    even = 0
-    for i in range(nb):
-        h = work[bounds[i]:bounds[i + 1]]
-        even += int(((((h >> 14) // 48) & 1) == 0).sum().item())
-    keys_phase = (even, n_ins - even)  # insert keys handled by even / odd launches
 
     steps = args.steps
     n_q = n_ins + n_dis
@@ -186,8 +225,8 @@ def main():
     if c_get:
         cands["bulk_get"] = (ms_get, c_get, QUERY_BYTES * n_q * steps)
     ms_rb, c_rb = kern["rebuild_phase"]
-    if c_rb:
-        cands["rebuild_phase"] = (ms_rb, c_rb, INSERT_BYTES * n_ins * steps)
+    if c_rb:  # insert and delete keys both pass through the rebuild phases
+        cands["rebuild_phase"] = (ms_rb, c_rb, INSERT_BYTES * (n_ins + half.numel()) * steps)
     dom = max(cands, key=lambda k: cands[k][0])
     d_ms, d_cnt, d_bytes = cands[dom]
     achieved = d_bytes / (d_ms / 1e3) / 1e9
@@ -211,11 +250,15 @@ def main():
         "config": {
             "workload": (f"GQF bulk API: 2^{nbits} slots, r=8, bulk_insert {n_ins} splitmix64(seed={args.seed}) "
                          f"keys in batches of 2^{args.batch_log2}, then bulk_get of the inserted keys and of "
-                         f"2^{nbits} disjoint keys"),
+                         f"2^{nbits} disjoint keys, then bulk_delete of a seeded half"),
             "nbits": nbits, "load": args.load, "batch": batch, "inserts": n_ins,
             "queries": n_q, "seed": args.seed, "disjoint": f"splitmix64(seed={args.seed}) outputs {n_ins}..",
+            "delete_half": "bulk_delete of key i iff splitmix64(seed=7) output i is odd, batches of the same size",
         },
         "inserts_mops": round(ins_rate / 1e6, 3),
+        "deletes_mops": round(half.numel() * steps / (del_ms / 1e3) / 1e6, 3),
+        "delete_batch_ms_p50": round(float(np.percentile(np.array(del_lat), 50)), 4),
+        "delete_batch_ms_p99": round(float(np.percentile(np.array(del_lat), 99)), 4),
         "queries_mops": round(q_rate / 1e6, 3),
         "insert_batch_ms_p50": round(float(np.percentile(lat, 50)), 4),
         "insert_batch_ms_p99": round(float(np.percentile(lat, 99)), 4),
@@ -239,10 +282,10 @@ def main():
         },
         "checks": {"all_inserted_found": present_ok, "fp_hits": fp,
                    "fp_rate": round(fp / n_dis, 6), "status": status,
-                   "rebuild_keys_even_odd": keys_phase},
+                   "kept_after_half_delete_found": kept_ok, "deleted": int(half.numel())},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_nbits)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_nbits, args.seed)
     f.close()
     print(json.dumps(out), flush=True)
 

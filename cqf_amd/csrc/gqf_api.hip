@@ -237,7 +237,11 @@ extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
   GQF_CHECK(hipMemset(h.blocks, 0, bytes));  // qf_malloc memsets the image (src/gqf.cu:2081)
   dev_alloc(&h.status, 1);
   GQF_CHECK(hipMemset(h.status, 0, sizeof(uint32_t)));
-  if (!bulk_config) {
+  // the reference allocates the point-API locks only when !bulk_config
+  // (src/gqf.cu:2166-2190); they are always allocated here (8 MB at 2^28
+  // slots) so a point call on a bulk-configured filter cannot fault
+  (void)bulk_config;
+  {
     const size_t nl = (size_t)h.num_locks * GQF_REF_LOCK_DIST;
     dev_alloc(&h.locks, nl);
     GQF_CHECK(hipMemset(h.locks, 0, nl * sizeof(uint32_t)));
@@ -298,24 +302,44 @@ extern "C" uint64_t bulk_get_misses_wrapper(QF* qf, uint64_t* vals, uint64_t nva
   return count_misses(qf, vals, nvals);
 }
 
+// point_query of every key without locks, number not found
+// (point_bulk_get + point_get_wrapper, src/gqf.cu:3112-3150); synchronises.
 extern "C" uint64_t point_get_wrapper(QF* qf, uint64_t* hashes, uint64_t nitems) {
-  // point_query(key % range) == 0 for each key (src/gqf.cu:3112-3150); the
-  // invertible hash sees only key & (range - 1), so this is the miss count.
-  return count_misses(qf, hashes, nitems);
+  if (nitems == 0) return 0;
+  Filter* f = lookup(qf);
+  if (!f) return 0;
+  ensure_workspace(f, 0);
+  GQF_CHECK(hipMemsetAsync(f->ws.misses, 0, sizeof(unsigned long long), 0));
+  gqf_launch_point_query(f->dev, hashes, nitems, 0, nullptr, 0, f->ws.misses, 0);
+  unsigned long long h = 0;
+  GQF_CHECK(hipMemcpy(&h, f->ws.misses, sizeof(h), hipMemcpyDeviceToHost));
+  return (uint64_t)h;
 }
 
-// The reference kernel runs one locked point_insert per key
-// (src/gqf.cu:3153-3162).  The image a set of inserts leaves is the same in
-// any order, so the wrapper applies the keys as one bulk insertion of a copy
-// (the caller's array is left as it was, as the reference leaves it).
+// One locked point_insert per key (the point_bulk_insert kernel,
+// src/gqf.cu:3153-3162); synchronises.
 extern "C" void point_bulk_insert_wrapper(QF* qf, uint64_t* hashes, uint64_t nitems) {
-  if (nitems == 0) return;
-  Filter* f = lookup(qf);
-  if (!f) return;
-  ensure_workspace(f, nitems);
-  GQF_CHECK(hipMemcpyAsync(f->ws.upos, hashes, nitems * sizeof(uint64_t), hipMemcpyDeviceToDevice, 0));
-  mutate(qf, nitems, f->ws.upos, 0, kModeInsert, false, 0);
+  if (nitems == 0 || !lookup(qf)) return;
+  gqf_launch_point_insert(qf, hashes, nitems, 0, nullptr, 0, 0);
   GQF_CHECK(hipDeviceSynchronize());
+}
+
+extern "C" void point_insert_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, int32_t* rets,
+                                   void* stream) {
+  if (nvals == 0 || !lookup(qf)) return;
+  gqf_launch_point_insert(qf, keys, nvals, flags, rets, 0, (hipStream_t)stream);
+}
+
+extern "C" void point_insert_not_exists_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags,
+                                              int32_t* rets, void* stream) {
+  if (nvals == 0 || !lookup(qf)) return;
+  gqf_launch_point_insert(qf, keys, nvals, flags, rets, 1, (hipStream_t)stream);
+}
+
+extern "C" void point_query_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, uint64_t* counts,
+                                  int concurrent, void* stream) {
+  if (nvals == 0 || !lookup(qf)) return;
+  gqf_launch_point_query(qf, keys, nvals, flags, counts, concurrent, nullptr, (hipStream_t)stream);
 }
 
 extern "C" void bulk_insert_async(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, void* stream) {

@@ -32,6 +32,12 @@ void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsig
 // device layout -> the reference's packed 81-byte image (nblocks * 81 bytes)
 void gqf_launch_export(const QF* qf_dev, uint64_t nblocks, uint8_t* out, hipStream_t s);
 
+// point API drivers: one lane per key (rets / counts / misses may be null)
+void gqf_launch_point_insert(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_t flags, int32_t* rets,
+                             int not_exists, hipStream_t s);
+void gqf_launch_point_query(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_t flags, uint64_t* counts,
+                            int concurrent, unsigned long long* misses, hipStream_t s);
+
 void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t skip, hipStream_t s);
 
 // Per-kernel event timing (off by default).  When enabled, the launchers

@@ -6,8 +6,8 @@
 // blocks plus a 16-block push window into the next region, one lane per
 // block, and applies ALL of the region's keys in one pass:
 //
-//   1. the window's 81-byte blocks are copied HBM -> LDS with 16-byte loads
-//      (the region starts at 48*81*r bytes, a multiple of 16) and the
+//   1. lane j loads window block j's 128-byte line with 16-byte loads: the
+//      header stays in registers, the 64 slots go to LDS row j; the
 //      region's sorted items are cached in LDS as 20-bit relative hashes;
 //   2. each lane walks its block's runs in the old layout (starting from the
 //      stored offset), merges the block's items counter by counter, and
@@ -17,11 +17,13 @@
 //      max(quotient, end of previous run + 1) -- the canonical CQF layout;
 //   4. the push into the next region's first blocks (a pure shift of their
 //      runs) ends at the first block whose end-before is unchanged;
-//   5. the new image is assembled in LDS and stored with 16-byte writes.
+//   5. the new slots are assembled in LDS rows and every rewritten block's
+//      line is stored with 16-byte writes (header, exact offset, slots).
 //
 // The result is the image the reference's sequential inserts and removes
-// produce (counters in encode_counter form, offsets saturated at 255),
-// because a CQF image is a function of the multiset of hashes alone.
+// produce (counters in encode_counter form; the exported offset byte is the
+// exact offset saturated at 255), because a CQF image is a function of the
+// multiset of hashes alone.
 //
 // Regions run in two phases (even, then odd) so a push never races the
 // wave that owns the next region.  A region whose push is not absorbed in
@@ -43,7 +45,6 @@ enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 
 template <int W>
 struct Smem {
-  // whole 64-chunk rows so staging can load and store branch-free
   uint32_t sin[W * 17];   // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
   uint32_t sout[W * 17];  // new slots, same rows
   uint64_t run_in[W];
@@ -84,29 +85,6 @@ __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   b = b + b2;
 }
 
-// counter emission in slot order (the bytes encode_counter writes,
-// src/gqf.cu:1216-1272)
-template <typename Put>
-__device__ __forceinline__ uint32_t emit_counter(uint64_t rem, uint64_t count, Put put) {
-  if (count == 0) return 0;
-  if (count == 1) { put(0, rem); return 1; }
-  if (count == 2) { put(0, rem); put(1, rem); return 2; }
-  if (count == 3) {  // r 0 r, or 0 0 0 for r == 0
-    put(0, rem);
-    put(1, 0);
-    put(2, rem);
-    return 3;
-  }
-  gqf_digits g;
-  gqf_counter_digits(rem, count, &g);
-  uint32_t k = 0;
-  put(k++, rem);
-  if (rem && g.pad0) put(k++, 0);
-  for (int i = 0; i < g.n; i++) put(k++, g.d[i]);
-  if (rem == 0) put(k++, 0);
-  put(k++, rem);
-  return k;
-}
 
 // The region's items: cached relative hashes in LDS, or (when the region
 // holds more than kItemCap) read from global memory.
@@ -176,11 +154,11 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
       len += clen;
       next_old();
     } else if (!have_old || n_rem < o_rem) {
-      if (mode == kInsert) len += emit_counter(n_rem, n_cnt, put);
+      if (mode == kInsert) len += gqf_emit_counter(n_rem, n_cnt, put);
       next_new();
     } else {
       const uint64_t c = mode == kInsert ? o_cnt + n_cnt : (o_cnt > n_cnt ? o_cnt - n_cnt : 0);
-      len += emit_counter(n_rem, c, put);
+      len += gqf_emit_counter(n_rem, c, put);
       next_old();
       next_new();
     }

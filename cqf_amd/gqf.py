@@ -32,6 +32,7 @@ EXPORTS = [
     "qf_malloc_device", "qf_destroy_device",
     "bulk_insert", "bulk_insert_reduce", "bulk_delete", "bulk_get", "bulk_get_misses_wrapper",
     "point_get_wrapper", "point_bulk_insert_wrapper",
+    "point_insert_async", "point_insert_not_exists_async", "point_query_async",
     "qf_get_info", "qf_get_status", "qf_clear_status",
     "qf_export_image", "qf_import_image", "qf_export_image_device", "qf_import_image_device",
     "qf_reserve", "qf_reset_device", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
@@ -71,6 +72,9 @@ def load_library(path=LIB_PATH):
     L.point_get_wrapper.argtypes = [_vp, _vp, _u64]
     L.point_get_wrapper.restype = _u64
     L.point_bulk_insert_wrapper.argtypes = [_vp, _vp, _u64]
+    L.point_insert_async.argtypes = [_vp, _u64, _vp, ctypes.c_uint8, _vp, _vp]
+    L.point_insert_not_exists_async.argtypes = [_vp, _u64, _vp, ctypes.c_uint8, _vp, _vp]
+    L.point_query_async.argtypes = [_vp, _u64, _vp, ctypes.c_uint8, _vp, ctypes.c_int, _vp]
     L.qf_get_info.argtypes = [_vp, ctypes.POINTER(GqfInfo)]
     L.qf_get_info.restype = ctypes.c_int
     L.qf_get_status.argtypes = [_vp]
@@ -178,6 +182,28 @@ class GQF:
 
     def point_bulk_insert(self, keys):
         self._lib.point_bulk_insert_wrapper(self.handle, _ptr(keys), keys.numel())
+
+    # -- device point API over an array of keys (one lane per key)
+    def point_insert(self, keys, flags=0, stream=None):
+        """point_insert per key; returns an int32 tensor of qf_returns."""
+        import torch
+        rets = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+        self._lib.point_insert_async(self.handle, keys.numel(), _ptr(keys), flags, _ptr(rets), _stream_ptr(stream))
+        return rets
+
+    def point_insert_not_exists(self, keys, flags=0, stream=None):
+        import torch
+        rets = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+        self._lib.point_insert_not_exists_async(self.handle, keys.numel(), _ptr(keys), flags, _ptr(rets),
+                                                _stream_ptr(stream))
+        return rets
+
+    def point_query(self, keys, flags=0, concurrent=False, stream=None):
+        import torch
+        counts = torch.empty(keys.numel(), dtype=torch.int64, device=keys.device)
+        self._lib.point_query_async(self.handle, keys.numel(), _ptr(keys), flags, _ptr(counts),
+                                    1 if concurrent else 0, _stream_ptr(stream))
+        return counts
 
     def reset(self, stream=None):
         return self._lib.qf_reset_device(self.handle, _stream_ptr(stream))

@@ -54,8 +54,8 @@ enum qf_returns { QF_ITEM_INSERTED, QF_ITEM_FOUND, QF_FULL };
 /* ---- lifecycle ----------------------------------------------------- */
 
 /* replaces include/gqf.cuh:87 (src/gqf.cu:2123-2221): 2^nbits slots, 8-bit
- * remainders, invertible hash, seed 0.  bulk_config = false also allocates
- * the point-API region locks. */
+ * remainders, invertible hash, seed 0.  The point-API region locks are
+ * allocated whatever bulk_config says. */
 void qf_malloc_device(QF **qf, int nbits, bool bulk_config);
 /* replaces include/gqf.cuh:88 (src/gqf.cu:2225-2269) */
 void qf_destroy_device(QF *qf);
@@ -80,8 +80,17 @@ uint64_t bulk_get_misses_wrapper(QF *qf, uint64_t *vals, uint64_t nvals);
 uint64_t point_get_wrapper(QF *qf, uint64_t *hashes, uint64_t nitems);
 /* replaces the point_bulk_insert kernel launch of include/gqf.cuh:136
  * (src/gqf.cu:3153-3162; launched by include/point_wrapper.cuh:63): one
- * locked point_insert per key. */
+ * locked point_insert per key; synchronises. */
 void point_bulk_insert_wrapper(QF *qf, uint64_t *hashes, uint64_t nitems);
+/* the device point functions (gqf_device.h) over a device array of keys,
+ * one lane per key, stream-ordered; rets[i] / counts[i] receive each
+ * call's return value (either may be NULL).  point_insert_async is the
+ * point_bulk_insert kernel (src/gqf.cu:3153-3162) with return codes. */
+void point_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, int32_t *rets, void *stream);
+void point_insert_not_exists_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, int32_t *rets,
+                                   void *stream);
+void point_query_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, uint64_t *counts, int concurrent,
+                       void *stream);
 
 /* ---- additions (no reference counterpart) -------------------------- */
 
@@ -89,7 +98,7 @@ typedef struct gqf_info {
   uint64_t nslots, xnslots, nblocks, key_bits, key_remainder_bits, bits_per_slot;
   uint64_t range, num_locks, nregions;
   uint64_t total_size_in_bytes;  /* bytes of the 81-byte-block image (as the reference's) */
-  uint64_t device_bytes;         /* bytes of the device table (84 per block + alignment) */
+  uint64_t device_bytes;         /* bytes of the device table: 128 per block + 4096 slack */
 } gqf_info;
 
 int qf_get_info(QF *qf, gqf_info *out);
