@@ -13,6 +13,23 @@
  *   bytes 9..16   runends     (u64, little endian)
  *   bytes 17..80  slots[64]   (u8 remainders, QF_BITS_PER_SLOT = 8)
  * (include/gqf_int.cuh:38-57).
+ *
+ * How each part is pinned before it is trusted (tests/test_oracle.py; the
+ * reference ships no golden vectors and its host path needs cuda.h and
+ * Thrust, so it is not built here):
+ *   hash_64 / hash_64i   multiplicative restatement, bijectivity, inverse
+ *   counter codec        encoder written from the format comment; decode of
+ *                        every encoding; insert1 increments reach it
+ *   insert / counted insert / remove images
+ *                        a layout builder written from the format's
+ *                        invariants alone (tests/spec_model.py); full delete
+ *                        gives the zero image
+ *   counts               brute-force masked multisets
+ *   sizing               closed-form table
+ *   insert1_if_not_exists  return codes and returnedVal on fixed cases;
+ *                        PARITY UNPINNED for its slot-wise search on runs
+ *                        holding multi-slot counters (restated literally from
+ *                        src/gqf.cu:1405-1440, no independent check exists)
  */
 #ifndef CQF_ORACLE_H
 #define CQF_ORACLE_H
