@@ -571,6 +571,7 @@ __device__ static inline void unlock(const QF& f, uint64_t idx) {
 template <typename Body>
 __device__ static inline void with_region_locks(const QF& f, uint64_t idx, Body body) {
   bool done = false;
+  uint32_t backoff = 1;
   while (__any(!done)) {
     if (!done) {
       if (try_lock(f, idx)) {
@@ -585,7 +586,11 @@ __device__ static inline void with_region_locks(const QF& f, uint64_t idx, Body 
           unlock(f, idx);
         }
       }
-      if (!done) __builtin_amdgcn_s_sleep(2);
+      if (!done) {
+        // exponential backoff keeps spinning lanes off the lock words
+        for (uint32_t k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(8);
+        backoff = backoff < 64 ? 2 * backoff : 64;
+      }
     }
   }
 }

@@ -23,7 +23,7 @@ def _ms(hashes):
     return ms
 
 
-@pytest.mark.parametrize("nbits,load,seed", [(12, 0.9, 1), (16, 0.9, 2), (18, 0.95, 3)])
+@pytest.mark.parametrize("nbits,load,seed", [(12, 0.9, 1), (16, 0.9, 2), (16, 0.95, 3)])
 def test_point_bulk_insert_matches_bulk_image(gpu, oracle, nbits, load, seed):
     from cqf_amd.gqf import GQF, to_device_u64
     n = int(load * (1 << nbits))
