@@ -55,6 +55,7 @@ struct Workspace {
 struct Filter {
   QF host;           // host mirror of the device descriptor
   QF* dev = nullptr; // the public handle
+  int sort_digit = 0;  // radix digit bits of the bulk sort (0: chosen from the key bits)
   Workspace ws;
   std::mutex mu;     // serialises workspace growth
 };
@@ -101,9 +102,11 @@ void ensure_workspace(Filter* f, uint64_t n) {
   if (!w.totals) {
     dev_alloc(&w.totals, 1024);
     dev_alloc(&w.m_dev, 1);
-    dev_alloc(&w.region_start, f->host.nregions + 2);
-    dev_alloc(&w.wide_flags, f->host.nregions + 1);
-    GQF_CHECK(hipMemset(w.wide_flags, 0, f->host.nregions + 1));
+    // sized for the smallest tunable region, so retuning never reallocates
+    const uint64_t maxreg = (f->host.nblocks + GQF_REGION_BLOCKS_MIN - 1) / GQF_REGION_BLOCKS_MIN;
+    dev_alloc(&w.region_start, maxreg + 2);
+    dev_alloc(&w.wide_flags, maxreg + 1);
+    GQF_CHECK(hipMemset(w.wide_flags, 0, maxreg + 1));
     dev_alloc(&w.wide_count, 1);
     GQF_CHECK(hipMemset(w.wide_count, 0, sizeof(uint32_t)));
     dev_alloc(&w.misses, 1);
@@ -129,7 +132,7 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   Workspace& w = f->ws;
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
   const int bits = hash_it ? (int)f->host.key_bits : 64;
-  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : 0, f->host.range - 1, bits, w.hist, w.totals, s);
+  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : 0, f->host.range - 1, bits, f->sort_digit, w.hist, w.totals, s);
   const uint64_t* items = keys;
   const uint64_t* counts = nullptr;
   const uint64_t* n_dev = nullptr;
@@ -231,7 +234,8 @@ extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
   h.bits_per_slot = 8;
   h.range = h.nslots << h.key_remainder_bits;
   h.num_locks = h.xnslots / GQF_REF_LOCK_SLOTS + 2;
-  h.nregions = (h.nblocks + GQF_REGION_BLOCKS - 1) / GQF_REGION_BLOCKS;
+  h.region_blocks = GQF_REGION_BLOCKS;
+  h.nregions = (h.nblocks + h.region_blocks - 1) / h.region_blocks;
   const size_t bytes = gqf_table_bytes(h.nblocks);
   dev_alloc(&h.blocks, bytes);
   GQF_CHECK(hipMemset(h.blocks, 0, bytes));  // qf_malloc memsets the image (src/gqf.cu:2081)
@@ -378,6 +382,8 @@ extern "C" int qf_get_info(QF* qf, gqf_info* out) {
   out->range = h.range;
   out->num_locks = h.num_locks;
   out->nregions = h.nregions;
+  out->region_blocks = h.region_blocks;
+  out->sort_digit_bits = (uint64_t)f->sort_digit;
   out->total_size_in_bytes = h.nblocks * GQF_REF_BLOCK_BYTES;  // the reference's image size
   out->device_bytes = gqf_table_bytes(h.nblocks);
   return 0;
@@ -475,6 +481,26 @@ extern "C" int qf_reset_device(QF* qf, void* stream) {
   return e == hipSuccess ? 0 : -3;
 }
 
+// Tuning knobs (0 keeps the current value): blocks per rebuild region and
+// radix digit bits of the bulk sort.  Takes effect for later calls on the
+// stream; synchronises.
+extern "C" int qf_set_tuning(QF* qf, uint32_t region_blocks, uint32_t sort_digit_bits) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  if (region_blocks && (region_blocks < GQF_REGION_BLOCKS_MIN || region_blocks > GQF_REGION_BLOCKS_MAX)) return -2;
+  if (sort_digit_bits && (sort_digit_bits < 6 || sort_digit_bits > 10)) return -2;
+  GQF_CHECK(hipDeviceSynchronize());
+  if (sort_digit_bits) f->sort_digit = (int)sort_digit_bits;
+  if (region_blocks) {
+    f->host.region_blocks = region_blocks;
+    f->host.nregions = (f->host.nblocks + region_blocks - 1) / region_blocks;
+    hipError_t e = hipMemcpy(f->dev, &f->host, sizeof(QF), hipMemcpyHostToDevice);
+    note_error(e, "qf_set_tuning");
+    if (e != hipSuccess) return -3;
+  }
+  return 0;
+}
+
 extern "C" int qf_reserve(QF* qf, uint64_t max_batch) {
   Filter* f = lookup(qf);
   if (!f) return -1;
@@ -524,9 +550,14 @@ void recycle_all() {
 }
 }  // namespace
 
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 void gqf_prof_begin(const char* name, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  if (!g_prof_on) return;
+  if (!g_prof_on || capturing(s)) return;
   ProfRec r;
   r.name = name;
   r.a = take_event();
@@ -537,13 +568,20 @@ void gqf_prof_begin(const char* name, hipStream_t s) {
 
 void gqf_prof_end(const char* name, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  if (!g_prof_on) return;
+  if (!g_prof_on || capturing(s)) return;
   auto it = g_prof_open.find(name);
   if (it == g_prof_open.end()) return;
   ProfRec& r = g_prof[it->second];
   r.b = take_event();
   GQF_CHECK(hipEventRecord(r.b, s));
   g_prof_open.erase(it);
+}
+
+bool gqf_filter_host(QF* handle, QF* out) {
+  Filter* f = lookup(handle);
+  if (!f) return false;
+  *out = f->host;
+  return true;
 }
 
 // Turns per-kernel event timing on (clearing earlier records) or off.

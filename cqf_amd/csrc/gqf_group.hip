@@ -1,0 +1,439 @@
+// gqf_group.hip -- many filters driven by shared launches, and the
+// autotuner of the per-filter knobs.
+//
+// No counterpart in the reference (it inserts one filter per call, each a
+// chain of host-synchronised launches).  A group holds existing filters.  A
+// group insert takes one device array of keys cut into one segment per
+// filter and runs, on one stream:
+//   tag      hash every key with its filter's mask and put the filter index
+//            above the hash bits: (f << S) | hash;
+//   sort     one radix sort of all tagged keys (filters come out in
+//            segment order, each sorted by hash);
+//   rebuild  ONE persistent launch over every filter's regions, fed by a
+//            device work queue (gqf_rebuild.hip), plus a wide-redo tail
+//            that exits at once when nothing overflowed;
+//   untag    each segment is left holding its filter's ascending hashes,
+//            as bulk_insert leaves its keys.
+// The sequence is captured once into a hipGraph per shape (key pointer,
+// segment sizes, flags, tuning) and replayed.  A group get is one launch.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gqf.h"
+#include "gqf_internal.h"
+#include "gqf_launch.h"
+
+namespace {
+
+#define GCHECK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) fprintf(stderr, "libgqf: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+void dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (hipMalloc((void**)p, (n ? n : 1) * sizeof(T)) != hipSuccess) *p = nullptr;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) GCHECK(hipFree(p));
+  p = nullptr;
+}
+
+__device__ inline uint32_t seg_of(const uint64_t* __restrict__ seg, uint32_t nf, uint64_t i) {
+  uint32_t lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (seg[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace
+
+// keys[i] <- (f << S) | hash_f(keys[i]); a pre-hashed key at or past its
+// filter's range becomes (f << S) | (2^S - 1), past every region, and is
+// flagged (S is one above the widest filter then).
+extern "C" __global__ void __launch_bounds__(256)
+gqf_group_tag(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n, int is_hash) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t f = seg_of(g.seg, g.nf, i);
+    const uint64_t range = g.descs[f].range;
+    uint64_t h = keys[i];
+    if (!is_hash) {
+      h = gqf_hash64(h, range - 1);
+    } else if (h >= range) {
+      h = (1ULL << g.tag_shift) - 1;
+      atomicOr(g.descs[f].status, GQF_STATUS_BAD_HASH);
+    }
+    keys[i] = ((uint64_t)f << g.tag_shift) | h;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+gqf_group_untag(uint64_t* __restrict__ keys, uint64_t n, int tag_shift) {
+  const uint64_t m = (1ULL << tag_shift) - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] &= m;
+}
+
+// next epoch of the done flags (a kernel, so a captured graph replays it)
+extern "C" __global__ void gqf_group_reset(uint32_t* counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counters[1] = counters[1] + 1;
+}
+
+namespace {
+
+inline uint32_t flat_grid(uint64_t n) {
+  const uint64_t g = (n + 255) / 256;
+  const uint64_t cap = 256ULL * 64;
+  return (uint32_t)(g < cap ? (g ? g : 1) : cap);
+}
+
+struct Group {
+  std::vector<QF*> handles;
+  uint32_t nf = 0;
+  std::mutex mu;
+  // device tables
+  QF* d_descs = nullptr;
+  uint64_t* d_tab = nullptr;  // seg | even_base | odd_base | reg_base, each nf + 1
+  uint32_t* d_counters = nullptr;
+  uint32_t* d_done = nullptr;
+  uint8_t* d_wide = nullptr;
+  uint64_t flag_cap = 0;
+  // uploaded table contents (to skip unchanged uploads)
+  std::vector<QF> h_descs;
+  std::vector<uint64_t> h_tab;
+  // sort workspace
+  uint64_t* tmp = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* totals = nullptr;
+  uint64_t cap = 0;
+  // graph cache
+  bool use_graph = true;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> graph_key;
+};
+
+std::mutex g_groups_mu;
+std::map<gqf_group*, Group*> g_groups;
+
+Group* glookup(gqf_group* h) {
+  std::lock_guard<std::mutex> lk(g_groups_mu);
+  auto it = g_groups.find(h);
+  return it == g_groups.end() ? nullptr : it->second;
+}
+
+// Refreshes the device tables from the filters' current descriptors and
+// the given segments; fills `dev`.  Returns false on a bad argument.
+bool refresh(Group* g, const uint64_t* offsets, bool is_hash, gqf_group_dev* dev, uint64_t* n_out) {
+  const uint32_t nf = g->nf;
+  std::vector<QF> descs(nf);
+  std::vector<uint64_t> tab(4 * (size_t)(nf + 1), 0);
+  uint64_t* seg = tab.data();
+  uint64_t* ev = seg + (nf + 1);
+  uint64_t* od = ev + (nf + 1);
+  uint64_t* rb = od + (nf + 1);
+  uint64_t maxbits = 0;
+  for (uint32_t f = 0; f < nf; f++) {
+    if (!gqf_filter_host(g->handles[f], &descs[f])) return false;
+    if (offsets && offsets[f + 1] < offsets[f]) return false;
+    seg[f + 1] = offsets ? offsets[f + 1] : 0;
+    const uint64_t nr = descs[f].nregions;
+    ev[f + 1] = ev[f] + (nr + 1) / 2;
+    od[f + 1] = od[f] + nr / 2;
+    rb[f + 1] = rb[f] + nr;
+    if (descs[f].key_bits > maxbits) maxbits = descs[f].key_bits;
+  }
+  if (offsets && offsets[0] != 0) return false;
+  const int shift = (int)maxbits + (is_hash ? 1 : 0);
+  uint32_t fbits = 0;
+  while ((1ULL << fbits) < nf) fbits++;
+  if (shift + (int)fbits > 64) return false;
+  if (rb[nf] + 1 > g->flag_cap) {
+    GCHECK(hipDeviceSynchronize());
+    dfree(g->d_done);
+    dfree(g->d_wide);
+    g->flag_cap = rb[nf] + 1;
+    dalloc(&g->d_done, g->flag_cap);
+    dalloc(&g->d_wide, g->flag_cap);
+    if (!g->d_done || !g->d_wide) return false;
+    // done flags hold epochs; the epoch counter only grows, so 0 never matches
+    GCHECK(hipMemset(g->d_done, 0, g->flag_cap * sizeof(uint32_t)));
+    GCHECK(hipMemset(g->d_wide, 0, g->flag_cap));
+  }
+  const bool descs_changed = g->h_descs.size() != nf || memcmp(descs.data(), g->h_descs.data(), sizeof(QF) * nf) != 0;
+  const bool tab_changed = tab != g->h_tab;
+  if (descs_changed || tab_changed) {
+    // earlier calls may still read the tables on another stream
+    GCHECK(hipDeviceSynchronize());
+    if (descs_changed) GCHECK(hipMemcpy(g->d_descs, descs.data(), sizeof(QF) * nf, hipMemcpyHostToDevice));
+    if (tab_changed) GCHECK(hipMemcpy(g->d_tab, tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice));
+    g->h_descs = descs;
+    g->h_tab = tab;
+  }
+  dev->descs = g->d_descs;
+  dev->seg = g->d_tab;
+  dev->even_base = g->d_tab + (nf + 1);
+  dev->odd_base = g->d_tab + 2 * (nf + 1);
+  dev->reg_base = g->d_tab + 3 * (nf + 1);
+  dev->nf = nf;
+  dev->tag_shift = shift;
+  dev->counters = g->d_counters;
+  dev->done = g->d_done;
+  dev->wide = g->d_wide;
+  *n_out = seg[nf];
+  return true;
+}
+
+void ensure_sort_space(Group* g, uint64_t n) {
+  if (n <= g->cap) return;
+  GCHECK(hipDeviceSynchronize());
+  dfree(g->tmp);
+  dfree(g->hist);
+  g->cap = n < 4096 ? 4096 : n;
+  dalloc(&g->tmp, g->cap);
+  dalloc(&g->hist, gqf_sort_hist_words(g->cap));
+}
+
+void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t n, int is_hash, int mode,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
+  uint32_t fbits = 0;
+  while ((1ULL << fbits) < dev.nf) fbits++;
+  gqf_launch_sort(keys, g->tmp, n, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
+  hipLaunchKernelGGL(gqf_group_reset, dim3(1), dim3(64), 0, s, g->d_counters);
+  gqf_launch_group_rebuild(dev, keys, mode, gqf_group_rebuild_grid(), s);
+  hipLaunchKernelGGL(gqf_group_untag, dim3(flat_grid(n)), dim3(256), 0, s, keys, n, dev.tag_shift);
+}
+
+}  // namespace
+
+extern "C" int gqf_group_create(gqf_group** out, QF* const* filters, uint32_t nfilters) {
+  if (!out) return -1;
+  *out = nullptr;
+  if (!filters || nfilters == 0) return -2;
+  Group* g = new Group();
+  g->nf = nfilters;
+  for (uint32_t f = 0; f < nfilters; f++) {
+    QF tmp;
+    if (!gqf_filter_host(filters[f], &tmp)) {
+      delete g;
+      return -2;
+    }
+    g->handles.push_back(filters[f]);
+  }
+  dalloc(&g->d_descs, nfilters);
+  dalloc(&g->d_tab, 4 * (size_t)(nfilters + 1));
+  dalloc(&g->d_counters, 4);
+  dalloc(&g->totals, 1024);
+  if (!g->d_descs || !g->d_tab || !g->d_counters || !g->totals) {
+    delete g;
+    return -3;
+  }
+  GCHECK(hipMemset(g->d_counters, 0, 4 * sizeof(uint32_t)));
+  GCHECK(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+  gqf_group* h = (gqf_group*)g;
+  {
+    std::lock_guard<std::mutex> lk(g_groups_mu);
+    g_groups[h] = g;
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" void gqf_group_destroy(gqf_group* h) {
+  Group* g = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_groups_mu);
+    auto it = g_groups.find(h);
+    if (it == g_groups.end()) return;
+    g = it->second;
+    g_groups.erase(it);
+  }
+  GCHECK(hipDeviceSynchronize());
+  if (g->exec) GCHECK(hipGraphExecDestroy(g->exec));
+  if (g->cap_stream) GCHECK(hipStreamDestroy(g->cap_stream));
+  dfree(g->d_descs);
+  dfree(g->d_tab);
+  dfree(g->d_counters);
+  dfree(g->d_done);
+  dfree(g->d_wide);
+  dfree(g->tmp);
+  dfree(g->hist);
+  dfree(g->totals);
+  delete g;
+}
+
+extern "C" int gqf_group_set_graphs(gqf_group* h, int on) {
+  Group* g = glookup(h);
+  if (!g) return -1;
+  std::lock_guard<std::mutex> lk(g->mu);
+  g->use_graph = on != 0;
+  return 0;
+}
+
+extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_t* offsets, uint8_t flags,
+                                     void* stream) {
+  Group* g = glookup(h);
+  if (!g) return -1;
+  std::lock_guard<std::mutex> lk(g->mu);
+  const int is_hash = (flags & QF_KEY_IS_HASH) ? 1 : 0;
+  gqf_group_dev dev;
+  uint64_t n = 0;
+  if (!refresh(g, offsets, is_hash, &dev, &n)) return -2;
+  if (n == 0) return 0;
+  ensure_sort_space(g, n);
+  if (!g->tmp || !g->hist) return -3;
+  hipStream_t s = (hipStream_t)stream;
+  (void)gqf_group_rebuild_grid();  // occupancy query outside any capture
+  if (!g->use_graph) {
+    enqueue_insert(g, dev, keys, n, is_hash, 0, s);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+  }
+  std::vector<uint64_t> key = g->h_tab;
+  key.push_back((uint64_t)(uintptr_t)keys);
+  key.push_back((uint64_t)is_hash);
+  key.push_back((uint64_t)(uintptr_t)g->tmp);
+  key.push_back((uint64_t)(uintptr_t)g->d_done);
+  for (const QF& d : g->h_descs) key.push_back(d.region_blocks);
+  if (!g->exec || key != g->graph_key) {
+    if (g->exec) {
+      GCHECK(hipGraphExecDestroy(g->exec));
+      g->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    GCHECK(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+    enqueue_insert(g, dev, keys, n, is_hash, 0, g->cap_stream);
+    GCHECK(hipStreamEndCapture(g->cap_stream, &graph));
+    if (!graph) return -3;
+    hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    GCHECK(hipGraphDestroy(graph));
+    if (e != hipSuccess) {
+      g->exec = nullptr;
+      return -3;
+    }
+    g->graph_key = key;
+  }
+  return hipGraphLaunch(g->exec, s) == hipSuccess ? 0 : -3;
+}
+
+extern "C" int gqf_group_bulk_get(gqf_group* h, const uint64_t* vals, const uint64_t* offsets, uint64_t* returns,
+                                  void* stream) {
+  Group* g = glookup(h);
+  if (!g) return -1;
+  std::lock_guard<std::mutex> lk(g->mu);
+  gqf_group_dev dev;
+  uint64_t n = 0;
+  if (!refresh(g, offsets, 0, &dev, &n)) return -2;
+  gqf_launch_group_get(dev, vals, n, returns, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: for a table of 2^nbits slots loaded to `load` in one batch,
+// times bulk_insert for each candidate region size and sort digit on a
+// scratch filter (synthetic splitmix64 keys) and keeps the fastest.
+// Results are cached per (nbits, load rounded to 0.05).
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, std::pair<uint32_t, uint32_t>> g_tune_cache;
+}  // namespace
+
+extern "C" int gqf_autotune(int nbits, double load, uint32_t* region_blocks, uint32_t* sort_digit_bits) {
+  if (nbits < 8 || nbits > 32 || !(load > 0) || load > 1.0) return -2;
+  const int bucket = (int)lround(load * 20.0);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find({nbits, bucket});
+    if (it != g_tune_cache.end()) {
+      if (region_blocks) *region_blocks = it->second.first;
+      if (sort_digit_bits) *sort_digit_bits = it->second.second;
+      return 0;
+    }
+  }
+  QF* qf = nullptr;
+  qf_malloc_device(&qf, nbits, true);
+  if (!qf) return -3;
+  const uint64_t n = (uint64_t)((double)(1ULL << nbits) * bucket / 20.0);
+  uint64_t* src = nullptr;
+  uint64_t* work = nullptr;
+  dalloc(&src, n);
+  dalloc(&work, n);
+  hipEvent_t a, b;
+  GCHECK(hipEventCreate(&a));
+  GCHECK(hipEventCreate(&b));
+  gqf_fill_splitmix64(src, n, 0x7A11u + (uint64_t)nbits, 0, nullptr);
+  qf_reserve(qf, n);
+  static const uint32_t kRegions[] = {24, 32, 40, 48, 56};
+  static const uint32_t kDigits[] = {0, 8};
+  float best = 1e30f;
+  uint32_t best_r = GQF_REGION_BLOCKS, best_d = 0;
+  for (uint32_t d : kDigits) {
+    for (uint32_t r : kRegions) {
+      qf_set_tuning(qf, r, d);
+      float ms_sum = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        qf_reset_device(qf, nullptr);
+        GCHECK(hipMemcpy(work, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice));
+        GCHECK(hipEventRecord(a, nullptr));
+        bulk_insert_async(qf, n, work, 0, nullptr);
+        GCHECK(hipEventRecord(b, nullptr));
+        GCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        GCHECK(hipEventElapsedTime(&ms, a, b));
+        if (rep) ms_sum += ms;  // first run warms up
+      }
+      if (qf_get_status(qf) != 0) {
+        qf_clear_status(qf);
+        continue;
+      }
+      if (ms_sum < best) {
+        best = ms_sum;
+        best_r = r;
+        best_d = d;
+      }
+    }
+  }
+  GCHECK(hipEventDestroy(a));
+  GCHECK(hipEventDestroy(b));
+  dfree(src);
+  dfree(work);
+  qf_destroy_device(qf);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[{nbits, bucket}] = {best_r, best_d};
+  }
+  if (region_blocks) *region_blocks = best_r;
+  if (sort_digit_bits) *sort_digit_bits = best_d;
+  return 0;
+}
+
+// Tunes every filter of the group for the load its segment brings
+// (offsets as for gqf_group_bulk_insert); synchronises.
+extern "C" int gqf_group_autotune(gqf_group* h, const uint64_t* offsets) {
+  Group* g = glookup(h);
+  if (!g || !offsets) return -1;
+  for (uint32_t f = 0; f < g->nf; f++) {
+    QF d;
+    if (!gqf_filter_host(g->handles[f], &d)) return -2;
+    double load = (double)(offsets[f + 1] - offsets[f]) / (double)d.nslots;
+    if (load <= 0) continue;
+    if (load > 1.0) load = 1.0;
+    uint32_t r = 0, dg = 0;
+    int rc = gqf_autotune((int)(d.key_bits - 8), load, &r, &dg);
+    if (rc != 0) return rc;
+    if (qf_set_tuning(g->handles[f], r, dg) != 0) return -3;
+  }
+  return 0;
+}

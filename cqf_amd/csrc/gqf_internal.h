@@ -6,8 +6,9 @@
 
 #include "gqf_device.h"
 
-#define GQF_REGION_BLOCKS 48                       // rebuild region; + 16 push blocks = one wave64
-#define GQF_REGION_SLOTS (GQF_REGION_BLOCKS * GQF_SLOTS_PER_BLOCK)
+#define GQF_REGION_BLOCKS 48      // default rebuild region; + 16 push blocks = one wave64
+#define GQF_REGION_BLOCKS_MIN 16  // tunable range (the push window is 64 - region blocks)
+#define GQF_REGION_BLOCKS_MAX 56
 
 // status bits reported through qf_get_status()
 #define GQF_STATUS_NO_SPACE QF_STATUS_NO_SPACE                  // a run would end past xnslots; region left unchanged

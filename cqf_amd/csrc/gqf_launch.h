@@ -8,7 +8,8 @@
 // radix sort of n keys in place (tmp holds n keys); the first pass hashes
 // with `mask` when hash_it is set.  hist must hold gqf_sort_hist_words(n).
 uint64_t gqf_sort_hist_words(uint64_t n);
-void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits,
+// digit_bits: 6..10 fixes the radix digit (passes rounded up to even), 0 picks 4 passes
+void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits, int digit_bits,
                      uint32_t* hist, uint32_t* totals, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
@@ -45,3 +46,32 @@ void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t sk
 // the completed intervals of one kernel name.
 void gqf_prof_begin(const char* name, hipStream_t s);
 void gqf_prof_end(const char* name, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Filter groups (gqf_group.hip): several filters served by shared launches.
+
+// Device-side tables of a group, passed by value to its kernels.
+struct gqf_group_dev {
+  const QF* descs;            // nf filter descriptors (copies of each filter's)
+  const uint64_t* seg;        // nf + 1 offsets of each filter's keys
+  const uint64_t* even_base;  // nf + 1 prefix sums of the filters' even-region counts
+  const uint64_t* odd_base;   // nf + 1 prefix sums of the odd-region counts
+  const uint64_t* reg_base;   // nf + 1 prefix sums of the region counts (flag index)
+  uint32_t nf;
+  int tag_shift;              // hash bits below the filter tag
+  uint32_t* counters;         // [1] epoch of the done flags, [2] wide-redo count
+  uint32_t* done;             // per region: epoch in which it finished
+  uint8_t* wide;              // per region: redo with the wide window
+};
+
+// the persistent rebuild over every filter's regions (even items, then odd
+// items that wait for their even neighbours), then the wide redo tail
+void gqf_launch_group_rebuild(const gqf_group_dev& g, const uint64_t* items, int mode, uint32_t grid,
+                              hipStream_t s);
+uint32_t gqf_group_rebuild_grid();
+// returns[i] = count of vals[i] in the filter whose segment holds i
+void gqf_launch_group_get(const gqf_group_dev& g, const uint64_t* vals, uint64_t n, uint64_t* returns,
+                          hipStream_t s);
+// the host copy of a filter's descriptor (false if the handle is unknown)
+bool gqf_filter_host(QF* handle, QF* out);
+

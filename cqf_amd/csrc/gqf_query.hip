@@ -120,6 +120,26 @@ gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
   if ((threadIdx.x & 63) == 0 && mine) atomicAdd(misses, (unsigned long long)mine);
 }
 
+// One launch over the keys of a filter group: key i belongs to the filter
+// whose segment [seg[f], seg[f+1]) holds i.
+extern "C" __global__ void __launch_bounds__(256)
+gqf_group_get_kernel(gqf_group_dev g, const uint64_t* __restrict__ vals, uint64_t n, uint64_t* __restrict__ returns) {
+  __shared__ uint32_t rows[256 * 17];
+  uint32_t* row = rows + 17 * threadIdx.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = g.nf;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (g.seg[mid] <= i) lo = mid; else hi = mid;
+    }
+    const QF& qf = g.descs[lo];
+    QF q;
+    q.blocks = qf.blocks;
+    q.nblocks = qf.nblocks;
+    returns[i] = gqf_count_hash(q, gqf_hash64(vals[i], qf.range - 1), row);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -163,3 +183,12 @@ void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t sk
   if (n == 0) return;
   hipLaunchKernelGGL(gqf_splitmix64_kernel, dim3(probe_grid(n)), dim3(256), 0, s, out, n, seed, skip);
 }
+
+void gqf_launch_group_get(const gqf_group_dev& g, const uint64_t* vals, uint64_t n, uint64_t* returns,
+                          hipStream_t s) {
+  if (n == 0) return;
+  gqf_prof_begin("group_get", s);
+  hipLaunchKernelGGL(gqf_group_get_kernel, dim3(probe_grid(n)), dim3(256), 0, s, g, vals, n, returns);
+  gqf_prof_end("group_get", s);
+}
+

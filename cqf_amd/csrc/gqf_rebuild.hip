@@ -34,8 +34,7 @@
 
 namespace {
 
-constexpr int R = GQF_REGION_BLOCKS;  // 48 region blocks
-constexpr int kWinFast = 64;          // + 16 push blocks
+constexpr int kWinFast = 64;          // region blocks + push blocks
 constexpr int kWinWide = 256;         // fallback window (208 push blocks)
 constexpr int kItemCap = 320;         // items cached in LDS per region (fast path)
 constexpr int32_t kNegInf = INT32_MIN / 4;
@@ -368,19 +367,22 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
 // Region rebuild.  All 64 lanes of the (single-wave) workgroup call it.
 // Lane l handles window blocks l, l + 64, ...
 
+// `tag` is added to every hash of `items` (a filter group sorts all its
+// filters' hashes together, each tagged with its filter above the hash bits).
 template <int W>
 __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
-                              Smem<W>& sm) {
+                              Smem<W>& sm, uint64_t tag = 0) {
   constexpr int kRounds = W / 64;
   const int lane = lane_id();
-  const uint64_t B0 = r * R;
+  const int R = (int)qf.region_blocks;
+  const uint64_t B0 = r * (uint64_t)R;
   const int nwin = (int)min((uint64_t)W, qf.nblocks - B0);
   const int nreg = (int)min((uint64_t)R, qf.nblocks - B0);
   const int64_t win_base = (int64_t)(B0 * 64);
   const bool table_end_in_window = B0 + (uint64_t)nwin >= qf.nblocks;
   const uint32_t m = (uint32_t)(e - s);
-  const Items src{items, counts, s, B0 << 14, m <= (uint32_t)kItemCap};
+  const Items src{items, counts, s, tag + (B0 << 14), m <= (uint32_t)kItemCap};
 
   // ---- 1. stage the window and the items ----
   // lane j reads block j's line: header into registers, slots into LDS row j
@@ -710,7 +712,7 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   if (lane == 0) *wide_count = 0;
 }
 
-// region_start[r] = first index i with items[i] >= r * (48 * 64 * 256), for
+// region_start[r] = first index i with items[i] >= r * (region_blocks * 64 * 256), for
 // r in [0, nregions]; items at or above `range` fall past the last region.
 extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                                              uint64_t n_host, const uint64_t* __restrict__ n_dev,
@@ -720,13 +722,153 @@ extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const u
   if (r > qf.nregions) return;
   const uint64_t n = n_dev ? *n_dev : n_host;
   uint64_t lo = 0, hi = n;
-  const uint64_t bound = r == qf.nregions ? qf.range : (r * R) << 14;
+  const uint64_t bound = r == qf.nregions ? qf.range : (r * qf.region_blocks) << 14;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
     if (items[mid] < bound) lo = mid + 1; else hi = mid;
   }
   region_start[r] = (uint32_t)lo;
   if (r == qf.nregions && lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
+}
+
+// ---------------------------------------------------------------------------
+// Filter groups: every filter's regions in one persistent launch.
+//
+// Work items are all even regions of all filters, then all odd regions.
+// Workgroup w takes items w, w + G, w + 2G, ... (G = grid size, at most the
+// number of workgroups the GPU holds at once, so all of them are resident):
+// the item index is uniform by construction, with no atomics and no
+// broadcast between lanes.  An odd region waits until its even neighbours
+// (whose windows overlap its own) are done.  Every workgroup finishes its
+// even items before it reaches any odd item, and even items wait for
+// nothing, so each awaited item belongs to a resident workgroup that gets
+// to it: the wait always ends (it is bounded anyway and reported in the
+// status if the bound runs out).  Done flags carry the epoch of the call,
+// so they never need clearing.  Writes are published with an agent-scope
+// release before the flag and read after an acquire, since the two regions
+// may run on different XCDs.
+
+namespace {
+
+__device__ inline uint64_t lower_bound_u64(const uint64_t* __restrict__ a, uint64_t lo, uint64_t hi, uint64_t v) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// filter f of work item k: the last f with base[f] <= k
+__device__ inline uint32_t item_filter(const uint64_t* __restrict__ base, uint32_t nf, uint64_t k) {
+  uint32_t lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= k) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Bounded wait for a region's completion flag, executed by the whole wave
+// with a wave-uniform exit (readfirstlane of the polled value), so the
+// loop cannot be split between lanes; false if the bound ran out.
+__device__ inline bool wait_done(const uint32_t* flag, uint32_t epoch) {
+  for (uint32_t spin = 0; spin < (1u << 22); spin++) {
+    const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__builtin_amdgcn_readfirstlane(v) == epoch) return true;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  return false;
+}
+
+__device__ inline void region_hash_bounds(const gqf_group_dev& g, const QF& qf, uint32_t f, uint64_t r,
+                                          const uint64_t* __restrict__ items, uint64_t& s, uint64_t& e,
+                                          uint64_t& tag) {
+  // tag + hash, not tag | hash: the last bound, tag + range, reaches the
+  // next filter's tag when range = 2^tag_shift
+  tag = (uint64_t)f << g.tag_shift;
+  const uint64_t lo_h = tag + ((r * qf.region_blocks) << 14);
+  const uint64_t hi_h = r + 1 >= qf.nregions ? tag + qf.range : tag + (((r + 1) * qf.region_blocks) << 14);
+  s = lower_bound_u64(items, g.seg[f], g.seg[f + 1], lo_h);
+  e = lower_bound_u64(items, s, g.seg[f + 1], hi_h);
+}
+
+}  // namespace
+
+// Every value that steers the item loop is wave-uniform: the item index
+// comes from blockIdx, and the flag polls exit on readfirstlane of the
+// polled value.  (A workgroup is one wave, so its __syncthreads() is only a
+// wave barrier: it does not keep lanes in the same loop iteration, and a
+// per-lane loop condition would let the compiler split the lanes.)
+extern "C" __global__ void __launch_bounds__(64)
+gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
+  __shared__ Smem<kWinFast> sm;
+  const int lane = lane_id();
+  const uint32_t epoch = __builtin_amdgcn_readfirstlane(
+      __hip_atomic_load(&g.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const uint64_t n_even = g.even_base[g.nf];
+  const uint64_t total = n_even + g.odd_base[g.nf];
+  for (uint64_t idx = blockIdx.x; idx < total; idx += gridDim.x) {
+    const bool odd = idx >= n_even;
+    const uint64_t k = odd ? idx - n_even : idx;
+    const uint64_t* base = odd ? g.odd_base : g.even_base;
+    const uint32_t f = item_filter(base, g.nf, k);
+    const uint64_t r = 2 * (k - base[f]) + (odd ? 1 : 0);
+    const QF qf = g.descs[f];
+    const uint64_t flag = g.reg_base[f] + r;
+    if (odd) {
+      bool ok = wait_done(&g.done[flag - 1], epoch);
+      if (r + 1 < qf.nregions) ok = wait_done(&g.done[flag + 1], epoch) && ok;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (!ok && lane == 0) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+    }
+    uint64_t s, e, tag;
+    region_hash_bounds(g, qf, f, r, items, s, e, tag);
+    int res = kOk;
+    if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, tag);
+    __syncthreads();
+    if (lane == 0) {
+      if (res == kNeedWide) {
+        g.wide[flag] = 1;
+        atomicAdd(&g.counters[2], 1u);
+      } else if (res == kNoSpace) {
+        atomicOr(qf.status, GQF_STATUS_NO_SPACE);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_store(&g.done[flag], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Redoes flagged regions with the wide window, filter by filter in
+// ascending region order.  One workgroup; exits at once when none.
+extern "C" __global__ void __launch_bounds__(64)
+gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
+  __shared__ Smem<kWinWide> sm;
+  if (*(volatile uint32_t*)&g.counters[2] == 0) return;
+  const int lane = lane_id();
+  for (uint32_t f = 0; f < g.nf; f++) {
+    const QF qf = g.descs[f];
+    const uint64_t f0 = g.reg_base[f];
+    for (uint64_t base = 0; base < qf.nregions; base += 64) {
+      const uint64_t r = base + (uint64_t)lane;
+      uint64_t m = __ballot(r < qf.nregions && g.wide[f0 + r]);
+      while (m) {
+        const uint64_t rr = base + (uint64_t)__builtin_ctzll(m);
+        m &= m - 1;
+        uint64_t s, e, tag;
+        region_hash_bounds(g, qf, f, rr, items, s, e, tag);
+        const int res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, tag);
+        __syncthreads();
+        if (lane == 0) {
+          g.wide[f0 + rr] = 0;
+          if (res == kNeedWide) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+          else if (res == kNoSpace) atomicOr(qf.status, GQF_STATUS_NO_SPACE);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (lane == 0) g.counters[2] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -755,3 +897,30 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
     gqf_prof_end("rebuild_wide", s);
   }
 }
+
+// Workgroups the GPU holds at once for the persistent rebuild (all of the
+// grid must be resident: see gqf_group_rebuild).
+uint32_t gqf_group_rebuild_grid() {
+  static uint32_t grid = 0;
+  if (!grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gqf_group_rebuild, 64, 0);
+    if (per_cu < 1) per_cu = 1;
+    if (per_cu > 8) per_cu = 8;  // LDS allows 8 (19.5 KB each of 160 KB); stay within it
+    grid = (uint32_t)((cus > 0 ? cus : 1) * per_cu);
+  }
+  return grid;
+}
+
+void gqf_launch_group_rebuild(const gqf_group_dev& g, const uint64_t* items, int mode, uint32_t grid,
+                              hipStream_t s) {
+  gqf_prof_begin("group_rebuild", s);
+  hipLaunchKernelGGL(gqf_group_rebuild, dim3(grid ? grid : 1), dim3(64), 0, s, g, items, mode);
+  gqf_prof_end("group_rebuild", s);
+  gqf_prof_begin("rebuild_wide", s);
+  hipLaunchKernelGGL(gqf_group_wide, dim3(1), dim3(64), 0, s, g, items, mode);
+  gqf_prof_end("rebuild_wide", s);
+}
+

@@ -275,12 +275,16 @@ static inline uint32_t sort_ntiles(uint64_t n) { return (uint32_t)((n + kTile - 
 
 uint64_t gqf_sort_hist_words(uint64_t n) { return (uint64_t)kMaxRadix * (sort_ntiles(n) ? sort_ntiles(n) : 1); }
 
-void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits,
+void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits, int digit_bits,
                      uint32_t* hist, uint32_t* totals, hipStream_t s) {
   if (n == 0) return;
   // an even number of passes leaves the result in `keys`
   int passes, digit;
-  if (bits <= 40) {
+  if (digit_bits >= 6 && digit_bits <= 10) {
+    digit = digit_bits;
+    passes = (bits + digit - 1) / digit;
+    passes += passes & 1;
+  } else if (bits <= 40) {
     passes = 4;
     digit = (bits + 3) / 4;
     if (digit < 6) digit = 6;

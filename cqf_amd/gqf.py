@@ -35,15 +35,18 @@ EXPORTS = [
     "point_insert_async", "point_insert_not_exists_async", "point_query_async",
     "qf_get_info", "qf_get_status", "qf_clear_status",
     "qf_export_image", "qf_import_image", "qf_export_image_device", "qf_import_image_device",
-    "qf_reserve", "qf_reset_device", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
+    "qf_reserve", "qf_set_tuning", "qf_reset_device", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
     "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
+    "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
+    "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
 ]
 
 
 class GqfInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "nslots", "xnslots", "nblocks", "key_bits", "key_remainder_bits", "bits_per_slot",
-        "range", "num_locks", "nregions", "total_size_in_bytes", "device_bytes")]
+        "range", "num_locks", "nregions", "total_size_in_bytes", "device_bytes", "region_blocks",
+        "sort_digit_bits")]
 
 
 _lib = None
@@ -92,11 +95,27 @@ def load_library(path=LIB_PATH):
     L.qf_reset_device.restype = ctypes.c_int
     L.qf_reserve.argtypes = [_vp, _u64]
     L.qf_reserve.restype = ctypes.c_int
+    L.qf_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.qf_set_tuning.restype = ctypes.c_int
     L.qf_last_error.restype = ctypes.c_int
     L.gqf_profile_enable.argtypes = [ctypes.c_int]
     L.gqf_profile_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]
     L.gqf_profile_read.restype = ctypes.c_int
     L.gqf_fill_splitmix64.argtypes = [_vp, _u64, _u64, _u64, _vp]
+    L.gqf_group_create.argtypes = [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_uint32]
+    L.gqf_group_create.restype = ctypes.c_int
+    L.gqf_group_destroy.argtypes = [_vp]
+    L.gqf_group_bulk_insert.argtypes = [_vp, _vp, ctypes.POINTER(_u64), ctypes.c_uint8, _vp]
+    L.gqf_group_bulk_insert.restype = ctypes.c_int
+    L.gqf_group_bulk_get.argtypes = [_vp, _vp, ctypes.POINTER(_u64), _vp, _vp]
+    L.gqf_group_bulk_get.restype = ctypes.c_int
+    L.gqf_group_set_graphs.argtypes = [_vp, ctypes.c_int]
+    L.gqf_group_set_graphs.restype = ctypes.c_int
+    L.gqf_autotune.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32),
+                               ctypes.POINTER(ctypes.c_uint32)]
+    L.gqf_autotune.restype = ctypes.c_int
+    L.gqf_group_autotune.argtypes = [_vp, ctypes.POINTER(_u64)]
+    L.gqf_group_autotune.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -211,6 +230,17 @@ class GQF:
     def reserve(self, max_batch):
         return self._lib.qf_reserve(self.handle, max_batch)
 
+    def set_tuning(self, region_blocks=0, sort_digit_bits=0):
+        rc = self._lib.qf_set_tuning(self.handle, region_blocks, sort_digit_bits)
+        if rc != 0:
+            raise ValueError(f"qf_set_tuning({region_blocks}, {sort_digit_bits}) rc={rc}")
+        self.refresh_info()
+
+    def refresh_info(self):
+        info = GqfInfo()
+        self._lib.qf_get_info(self.handle, ctypes.byref(info))
+        self.info = {n: int(getattr(info, n)) for n, _ in GqfInfo._fields_}
+
     # -- status and image
     def status(self):
         return int(self._lib.qf_get_status(self.handle))
@@ -245,6 +275,64 @@ class GQF:
         rc = self._lib.qf_import_image_device(self.handle, _ptr(img), img.numel(), _stream_ptr(stream))
         if rc != 0:
             raise RuntimeError(f"qf_import_image_device rc={rc}")
+
+
+class GQFGroup:
+    """Several GQF filters served by shared launches (one device key array
+    cut into one segment per filter)."""
+
+    def __init__(self, filters, graphs=True):
+        self._lib = load_library()
+        self.filters = list(filters)
+        arr = (_vp * len(self.filters))(*[f.handle for f in self.filters])
+        h = ctypes.c_void_p()
+        rc = self._lib.gqf_group_create(ctypes.byref(h), arr, len(self.filters))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_create rc={rc}")
+        self.handle = h
+        self._lib.gqf_group_set_graphs(h, 1 if graphs else 0)
+
+    @staticmethod
+    def _offsets(offsets):
+        return (_u64 * len(offsets))(*[int(x) for x in offsets])
+
+    def bulk_insert(self, keys, offsets, flags=0, stream=None):
+        rc = self._lib.gqf_group_bulk_insert(self.handle, _ptr(keys), self._offsets(offsets), flags,
+                                             _stream_ptr(stream))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_bulk_insert rc={rc}")
+
+    def bulk_get(self, vals, offsets, returns=None, stream=None):
+        import torch
+        if returns is None:
+            returns = torch.empty_like(vals)
+        rc = self._lib.gqf_group_bulk_get(self.handle, _ptr(vals), self._offsets(offsets), _ptr(returns),
+                                          _stream_ptr(stream))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_bulk_get rc={rc}")
+        return returns
+
+    def autotune(self, offsets):
+        rc = self._lib.gqf_group_autotune(self.handle, self._offsets(offsets))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_autotune rc={rc}")
+        for f in self.filters:
+            f.refresh_info()
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            self._lib.gqf_group_destroy(self.handle)
+        self.handle = None
+
+
+def autotune(nbits, load):
+    """(region_blocks, sort_digit_bits) the autotuner picks for a table of
+    2^nbits slots loaded to `load` in one batch."""
+    r, d = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = load_library().gqf_autotune(nbits, load, ctypes.byref(r), ctypes.byref(d))
+    if rc != 0:
+        raise RuntimeError(f"gqf_autotune rc={rc}")
+    return r.value, d.value
 
 
 def fill_splitmix64(out, seed, skip=0, stream=None):

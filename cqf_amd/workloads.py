@@ -75,3 +75,68 @@ def zipf_workload(n_ids, n_draws, s=1.5, seed=4, distinct_bits=36, device="cpu",
         draws[b:b + m] = keys[ids]
         mult += torch.bincount(ids, minlength=n_ids)
     return keys, draws, mult
+
+
+def _bits(seed, n, skip, device):
+    return splitmix64(seed, n, skip=skip, device=device)
+
+
+def kmer_filter_params(nfilters=256, seed=5, q_lo=16, q_hi=24, load_lo=0.5, load_hi=0.95):
+    """Per filter: (log2 slots, load factor, k-mer instances, genome length).
+    q is uniform in [q_lo, q_hi], the load uniform in [load_lo, load_hi]
+    (splitmix64(seed) outputs 2f and 2f + 1); the genome is a quarter of
+    the k-mer instances long (about 4x coverage), at least 1000 bases."""
+    u = splitmix64(seed, 2 * nfilters).tolist()
+    out = []
+    for f in range(nfilters):
+        a = u[2 * f] % (1 << 64)
+        b = u[2 * f + 1] % (1 << 64)
+        q = q_lo + a % (q_hi - q_lo + 1)
+        load = load_lo + (load_hi - load_lo) * ((b >> 11) / float(1 << 53))
+        n = int(load * (1 << q))
+        out.append((q, load, n, max(1000, n // 4)))
+    return out
+
+
+def kmers(n, genome_len, seed, read_len=150, k=31, err_per_100=1, device="cpu", chunk_reads=1 << 16):
+    """n canonical k-mers (2-bit packed, min of forward and reverse
+    complement, 62 bits) from reads of read_len bases sampled uniformly from
+    a random genome of genome_len bases, each base substituted with
+    probability err_per_100 / 100.  All randomness is splitmix64(seed):
+    genome bases from outputs [0, ...), read starts from [2^40, ...), error
+    draws from [2^41, ...)."""
+    per_read = read_len - k + 1
+    n_reads = (n + per_read - 1) // per_read
+    words = (genome_len + 31) // 32
+    g = _bits(seed, words, 0, device)
+    shifts = torch.arange(0, 64, 2, device=device)
+    genome = (g[:, None] >> shifts) & 3  # the sign fill of >> never reaches the 2 kept bits
+    genome = genome.reshape(-1)[:genome_len].to(torch.int64)
+    w_fwd = torch.tensor([4 ** (k - 1 - i) for i in range(k)], dtype=torch.int64, device=device)
+    w_rev = torch.tensor([4 ** i for i in range(k)], dtype=torch.int64, device=device)
+    out = torch.empty(n_reads * per_read, dtype=torch.int64, device=device)
+    pos = torch.arange(read_len, device=device)
+    for r0 in range(0, n_reads, chunk_reads):
+        m = min(chunk_reads, n_reads - r0)
+        st = _srl(_bits(seed, m, (1 << 40) + r0, device), 1) % (genome_len - read_len + 1)
+        reads = genome[st[:, None] + pos]
+        e = _srl(_bits(seed, m * read_len, (1 << 41) + r0 * read_len, device), 1).reshape(m, read_len)
+        hit = (e % 100) < err_per_100
+        sub = (reads + 1 + (e >> 8) % 3) % 4
+        reads = torch.where(hit, sub, reads)
+        win = reads.unfold(1, k, 1)                      # m x per_read x k
+        fwd = (win * w_fwd).sum(-1)
+        rev = ((3 - win) * w_rev).sum(-1)
+        out[r0 * per_read:(r0 + m) * per_read] = torch.minimum(fwd, rev).reshape(-1)
+    return out[:n].contiguous()
+
+
+def kmer_workload(nfilters=256, seed=5, device="cpu", **kw):
+    """(params, keys, offsets): every filter's k-mer instances, concatenated
+    in filter order; filter f draws from splitmix64(seed * 1000003 + f)."""
+    params = kmer_filter_params(nfilters, seed, **kw)
+    parts, offsets = [], [0]
+    for f, (q, load, n, glen) in enumerate(params):
+        parts.append(kmers(n, glen, seed * 1000003 + f, device=device))
+        offsets.append(offsets[-1] + n)
+    return params, torch.cat(parts), offsets

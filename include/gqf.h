@@ -99,6 +99,8 @@ typedef struct gqf_info {
   uint64_t range, num_locks, nregions;
   uint64_t total_size_in_bytes;  /* bytes of the 81-byte-block image (as the reference's) */
   uint64_t device_bytes;         /* bytes of the device table: 128 per block + 4096 slack */
+  uint64_t region_blocks;        /* blocks per rebuild region (tuning) */
+  uint64_t sort_digit_bits;      /* radix digit of the bulk sort, 0 = automatic (tuning) */
 } gqf_info;
 
 int qf_get_info(QF *qf, gqf_info *out);
@@ -117,11 +119,42 @@ int qf_reset_device(QF *qf, void *stream);
 /* size the per-filter workspace for batches up to max_batch keys so that
  * later bulk calls allocate nothing */
 int qf_reserve(QF *qf, uint64_t max_batch);
+/* tuning (0 keeps a value): blocks per rebuild region, 16..56 (the push
+ * window is 64 minus that), and radix digit bits of the bulk sort, 6..10;
+ * the results do not depend on them, only the speed does */
+int qf_set_tuning(QF *qf, uint32_t region_blocks, uint32_t sort_digit_bits);
 /* stream-ordered variants of the bulk calls */
 void bulk_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_insert_reduce_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_delete_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_get_async(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns, void *stream);
+/* ---- filter groups and tuning (no reference counterpart) ---------- */
+
+/* A group of existing filters served by shared launches.  keys / vals are
+ * ONE device array cut into one segment per filter: segment f is
+ * [offsets[f], offsets[f+1]) (offsets: host array of nfilters + 1, offsets[0]
+ * = 0).  gqf_group_bulk_insert gives every segment bulk_insert semantics
+ * (the segment is left holding its filter's ascending hashes) with one
+ * radix sort and ONE persistent rebuild launch for all filters, captured
+ * into a hipGraph per shape and replayed; gqf_group_bulk_get fills
+ * returns[i] with the count of vals[i] in its segment's filter, in one
+ * launch.  Stream-ordered; a call whose segments or filter tuning differ
+ * from the previous one synchronises first. */
+typedef struct gqf_group gqf_group;
+int gqf_group_create(gqf_group **g, QF *const *filters, uint32_t nfilters);
+void gqf_group_destroy(gqf_group *g);
+int gqf_group_bulk_insert(gqf_group *g, uint64_t *keys, const uint64_t *offsets, uint8_t flags, void *stream);
+int gqf_group_bulk_get(gqf_group *g, const uint64_t *vals, const uint64_t *offsets, uint64_t *returns, void *stream);
+/* graphs on (default) or off (every call launches eagerly) */
+int gqf_group_set_graphs(gqf_group *g, int on);
+/* autotuner: times bulk_insert of a 2^nbits-slot table loaded to `load`
+ * in one batch for each candidate region size and sort digit, returns the
+ * fastest (cached per nbits and load rounded to 0.05); synchronises */
+int gqf_autotune(int nbits, double load, uint32_t *region_blocks, uint32_t *sort_digit_bits);
+/* gqf_autotune for every filter of the group at the load its segment
+ * brings, applied with qf_set_tuning */
+int gqf_group_autotune(gqf_group *g, const uint64_t *offsets);
+
 /* last HIP error seen by the library (0 = none); clears it */
 int qf_last_error(void);
 

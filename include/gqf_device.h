@@ -69,7 +69,8 @@ struct quotient_filter {
   uint64_t bits_per_slot;       // 8
   uint64_t range;               // 2^(q+8)
   uint64_t num_locks;           // xnslots / 8192 + 2 (src/gqf.cu:1996)
-  uint64_t nregions;            // ceil(nblocks / 64): even/odd rebuild regions
+  uint64_t nregions;            // ceil(nblocks / region_blocks): even/odd rebuild regions
+  uint64_t region_blocks;       // blocks per rebuild region (tunable, 16..56; default 48)
   uint8_t* blocks;              // nblocks * GQF_BLOCK_BYTES bytes (+ slack)
   uint32_t* locks;              // point-API region locks: num_locks words, 64 apart
   uint32_t* status;             // GQF_STATUS_* bits, sticky until cleared

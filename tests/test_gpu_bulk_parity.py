@@ -189,3 +189,31 @@ def test_misses_wrappers(gpu, oracle):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("region_blocks,digit", [(16, 6), (32, 8), (40, 10), (56, 0)])
+def test_tuning_does_not_change_results(gpu, oracle, region_blocks, digit):
+    """Region size and radix digit are speed knobs only: images and counts
+    stay the oracle's."""
+    from cqf_amd.gqf import to_device_u64, to_numpy_u64
+    nbits = 16
+    _say(f"case tuning R={region_blocks} digit={digit}")
+    keys = oracle.splitmix64(31, int(0.93 * (1 << nbits)))
+    g, o = _gpu_and_oracle(oracle, nbits)
+    try:
+        g.set_tuning(region_blocks, digit)
+        assert g.info["region_blocks"] == region_blocks
+        for part in np.array_split(keys, 3):
+            g.bulk_insert(to_device_u64(part))
+            o.bulk_insert(part.copy())
+        _assert_images_equal(g.export_image(), o.image(), "tuned insert")
+        half = keys[::2]
+        g.bulk_delete(to_device_u64(half))
+        o.bulk_delete(half.copy())
+        _assert_images_equal(g.export_image(), o.image(), "tuned delete")
+        got = to_numpy_u64(g.bulk_get(to_device_u64(keys)))
+        assert np.array_equal(got, o.bulk_get(keys))
+        assert g.status() == 0
+    finally:
+        g.close()
+        o.close()

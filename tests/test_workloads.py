@@ -35,3 +35,31 @@ def test_zipf_workload_distribution():
     # reproducible
     _, d2, m2 = W.zipf_workload(n_ids, n_draws, s=s, seed=4, distinct_bits=36)
     assert torch.equal(draws, d2) and torch.equal(mult, m2)
+
+
+def test_kmers_are_canonical_31mers():
+    km = W.kmers(20000, 5000, 11)
+    assert km.numel() == 20000 and int(km.min()) >= 0 and int(km.max()) < (1 << 62)
+
+    def dec(v):
+        return [(v >> (2 * (30 - i))) & 3 for i in range(31)]
+
+    def enc(bb):
+        return sum(x << (2 * (30 - i)) for i, x in enumerate(bb))
+
+    for v in km[:300].tolist():
+        b = dec(v)
+        assert v == min(enc(b), enc([3 - x for x in b[::-1]]))
+    # consecutive k-mers of one read overlap by 30 bases (one strand or the other)
+    a, b = dec(int(km[0])), dec(int(km[1]))
+    rc = lambda x: [3 - y for y in x[::-1]]
+    assert any(p[1:] == q[:-1] for p in (a, rc(a)) for q in (b, rc(b)))
+    # 4x coverage of a 5000-base genome: far fewer distinct k-mers than instances
+    assert torch.unique(km).numel() < 0.7 * km.numel()
+
+
+def test_kmer_params_ranges():
+    p = W.kmer_filter_params()
+    assert len(p) == 256
+    assert all(16 <= q <= 24 and 0.5 <= ld <= 0.95 and n == int(ld * (1 << q)) for q, ld, n, _ in p)
+    assert len({q for q, *_ in p}) == 9
