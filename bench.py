@@ -46,9 +46,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_q28.md"),
                     help="rocprofv3 counter summary holding the measured HBM bytes per launch")
-    ap.add_argument("--workload", choices=["bulk", "zipf", "groups"], default="bulk",
+    ap.add_argument("--workload", choices=["bulk", "zipf", "groups", "serve"], default="bulk",
                     help="bulk: uniform insert/get/delete (default); zipf: skewed bulk_insert_reduce; "
-                         "groups: 256 k-mer filters in one persistent launch")
+                         "groups: 256 k-mer filters in one persistent launch; serve: served request stream")
+    ap.add_argument("--serve-requests", type=int, default=6000)
+    ap.add_argument("--serve-rate", type=float, default=2.0e5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--groups-filters", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--zipf-ids-log2", type=int, default=24)
@@ -285,6 +290,70 @@ def groups_main(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_main(args):
+    """Served stream: 64 filters of 2^22..2^30 slots admitted against an HBM
+    budget; Poisson arrivals of 60% get / 30% insert / 10% delete requests,
+    lognormal sizes (median 2^14 keys, sigma 1.0, clipped to [1, 2^22]) of
+    uniform splitmix64(6) keys.  The same stream is served by the batching
+    loop (several streams, coalescing) and by plain per-request bulk calls
+    on one stream; ops/s counts keys."""
+    import numpy as np
+    import torch
+    from cqf_amd import serving as S
+    from cqf_amd import workloads as W
+
+    torch.cuda.set_device(0)
+    nbits, spec = W.serve_workload(args.serve_requests, nfilters=64, seed=6, q_lo=22, q_hi=30, median_log2=14,
+                                   sigma=1.0, max_log2=22, rate_per_s=args.serve_rate)
+    n_keys = sum(x[3] for x in spec)
+
+    def make():
+        return [S.Request(i, fid, op, W.splitmix64(6, n, skip=skip, device="cuda"), t)
+                for i, (t, fid, op, n, skip) in enumerate(spec)]
+
+    max_batch = 1 << 22
+    results = {}
+    for rep in range(args.warmup + 1):
+        reqs = make()
+        torch.cuda.synchronize()
+        srv = S.Server(budget_bytes=int(args.serve_budget_gb * 2 ** 30), n_streams=args.serve_streams,
+                       max_batch_keys=max_batch, max_inflight=args.serve_inflight)
+        for fid, nb in nbits.items():
+            srv.add_filter(fid, nb)
+        wall = srv.run(reqs)
+        torch.cuda.synchronize()
+        st = S.statuses(srv)
+        results["served"] = (wall, S.latency_stats(reqs), srv.batches, srv.evictions, srv.admissions,
+                             sum(1 for v in st.values() if v))
+        srv.close()
+        del srv
+        reqs = make()
+        torch.cuda.synchronize()
+        wall_p = S.run_plain(nbits, reqs, max_batch)
+        results["plain"] = (wall_p, S.latency_stats(reqs))
+    wall, lat, batches, ev, adm, bad_status = results["served"]
+    wall_p, lat_p = results["plain"]
+    out = {
+        "metric": "served ops/s (keys) and p50/p99 request latency, 64 resident filters, mixed get/insert/delete",
+        "value": round(n_keys / wall / 1e6, 3), "unit": "Mops/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": (f"{len(spec)} requests, Poisson {args.serve_rate:.0f}/s, 60/30/10 get/insert/delete, "
+                                "lognormal sizes median 2^14 sigma 1.0 clipped [1, 2^22], 64 filters q 22..30 "
+                                "(splitmix64 seed 6)"),
+                   "requests": len(spec), "keys": n_keys, "budget_gb": args.serve_budget_gb,
+                   "filters_slots": int(sum(1 << q for q in nbits.values()))},
+        "served": {"mops": round(n_keys / wall / 1e6, 3), "wall_s": round(wall, 4), "latency": lat,
+                   "batches": batches, "evictions": ev, "admissions": adm, "streams": args.serve_streams,
+                   "max_inflight": args.serve_inflight,
+                   "filters_with_status": bad_status},
+        "plain_per_request": {"mops": round(n_keys / wall_p / 1e6, 3), "wall_s": round(wall_p, 4),
+                              "latency": lat_p},
+        "speedup_vs_plain": round(wall_p / wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -300,6 +369,9 @@ def main():
         return
     if args.workload == "groups":
         groups_main(args)
+        return
+    if args.workload == "serve":
+        serve_main(args)
         return
 
     import numpy as np

@@ -47,6 +47,8 @@ struct Workspace {
   uint64_t* ucounts = nullptr;
   uint64_t* m_dev = nullptr;
   uint32_t* region_start = nullptr;  // nregions + 1
+  uint32_t* region_list = nullptr;   // non-empty regions of each parity
+  uint32_t* list_count = nullptr;    // 2 counts
   uint8_t* wide_flags = nullptr;     // nregions, kept zero by the kernels
   uint32_t* wide_count = nullptr;
   unsigned long long* misses = nullptr;
@@ -105,6 +107,8 @@ void ensure_workspace(Filter* f, uint64_t n) {
     // sized for the smallest tunable region, so retuning never reallocates
     const uint64_t maxreg = (f->host.nblocks + GQF_REGION_BLOCKS_MIN - 1) / GQF_REGION_BLOCKS_MIN;
     dev_alloc(&w.region_start, maxreg + 2);
+    dev_alloc(&w.region_list, 2 * ((maxreg + 1) / 2 + 1));
+    dev_alloc(&w.list_count, 2);
     dev_alloc(&w.wide_flags, maxreg + 1);
     GQF_CHECK(hipMemset(w.wide_flags, 0, maxreg + 1));
     dev_alloc(&w.wide_count, 1);
@@ -142,8 +146,10 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
     counts = w.ucounts;
     n_dev = w.m_dev;
   }
-  gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, s);
-  gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, w.region_start, mode, w.wide_flags, w.wide_count, s);
+  gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, w.region_list, w.list_count,
+                           s);
+  gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, n, w.region_start, w.region_list, w.list_count, mode,
+                     w.wide_flags, w.wide_count, s);
   GQF_CHECK(hipGetLastError());
 }
 
@@ -273,6 +279,8 @@ extern "C" void qf_destroy_device(QF* qf) {
   dev_free(f->ws.totals);
   dev_free(f->ws.m_dev);
   dev_free(f->ws.region_start);
+  dev_free(f->ws.region_list);
+  dev_free(f->ws.list_count);
   dev_free(f->ws.wide_flags);
   dev_free(f->ws.wide_count);
   dev_free(f->ws.misses);

@@ -17,13 +17,18 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
 void gqf_launch_rle(const uint64_t* keys, uint64_t n, uint32_t* tile_heads, uint64_t* ukeys,
                     uint64_t* upos, uint64_t* ucounts, uint64_t* m_dev, hipStream_t s);
 
-// region bounds of sorted items (count n, or *n_dev when n_dev is set)
+// region bounds of sorted items (count n, or *n_dev when n_dev is set),
+// plus the non-empty regions of each parity compacted into region_list
+// (2 lists of (nregions + 1) / 2 + 1 entries) with their counts
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
-                              const uint64_t* n_dev, uint32_t* region_start, hipStream_t s);
+                              const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
+                              uint32_t* list_count, hipStream_t s);
 
-// both even/odd phases (+ wide fallback passes) of a merge-rebuild
+// both even/odd phases (+ wide fallback passes) of a merge-rebuild over
+// the listed regions; n_items bounds the number of non-empty regions
 void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* items, const uint64_t* counts,
-                        const uint32_t* region_start, int mode, uint8_t* wide_flags, uint32_t* wide_count,
+                        uint64_t n_items, const uint32_t* region_start, const uint32_t* region_list,
+                        const uint32_t* list_count, int mode, uint8_t* wide_flags, uint32_t* wide_count,
                         hipStream_t s);
 
 void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uint64_t* returns, hipStream_t s);

@@ -657,14 +657,20 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
 // ---------------------------------------------------------------------------
 // kernels
 
+// One workgroup per non-empty region of one parity: workgroup i takes
+// region list[i] (the list gqf_region_bounds compacted; count[parity]
+// entries), so a small batch launches only as many workgroups as it has
+// regions with items.
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_phase(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                   const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
+                  const uint32_t* __restrict__ region_list, const uint32_t* __restrict__ list_count,
                   int parity, int mode, uint8_t* __restrict__ wide_flags,
                   uint32_t* __restrict__ wide_count) {
   __shared__ Smem<kWinFast> sm;
   const QF qf = *qfp;
-  const uint64_t r = 2ULL * blockIdx.x + (uint64_t)parity;
+  if (blockIdx.x >= list_count[parity]) return;
+  const uint64_t r = region_list[blockIdx.x];
   if (r >= qf.nregions) return;
   const uint64_t s = region_start[r], e = region_start[r + 1];
   if (s == e) return;
@@ -714,21 +720,53 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
 
 // region_start[r] = first index i with items[i] >= r * (region_blocks * 64 * 256), for
 // r in [0, nregions]; items at or above `range` fall past the last region.
+// Also appends each non-empty region r < nregions to region_list (at
+// parity r & 1, list stride `list_stride`), counting in list_count[parity]
+// (zeroed by gqf_region_lists_reset just before).
 extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                                              uint64_t n_host, const uint64_t* __restrict__ n_dev,
-                                             uint32_t* __restrict__ region_start) {
+                                             uint32_t* __restrict__ region_start, uint32_t* __restrict__ region_list,
+                                             uint32_t* __restrict__ list_count, uint64_t list_stride) {
   const QF qf = *qfp;
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r > qf.nregions) return;
+  const bool valid = r <= qf.nregions;
   const uint64_t n = n_dev ? *n_dev : n_host;
-  uint64_t lo = 0, hi = n;
-  const uint64_t bound = r == qf.nregions ? qf.range : (r * qf.region_blocks) << 14;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (items[mid] < bound) lo = mid + 1; else hi = mid;
+  auto lower = [&](uint64_t bound) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (items[mid] < bound) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  bool nonempty = false;
+  if (valid) {
+    const uint64_t lo = lower(r == qf.nregions ? qf.range : (r * qf.region_blocks) << 14);
+    region_start[r] = (uint32_t)lo;
+    if (r == qf.nregions) {
+      if (lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
+    } else {
+      nonempty = lower(r + 1 == qf.nregions ? qf.range : ((r + 1) * qf.region_blocks) << 14) > lo;
+    }
   }
-  region_start[r] = (uint32_t)lo;
-  if (r == qf.nregions && lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
+  // append, one atomic per wave and parity (the lanes of a wave hold
+  // consecutive regions, so even lanes are even regions when the wave's
+  // first region is even)
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  for (uint32_t p = 0; p < 2; p++) {
+    const uint64_t m = __ballot(nonempty && (uint32_t)(r & 1) == p);
+    if (!m) continue;
+    uint32_t base = 0;
+    if (lane == (uint32_t)__builtin_ctzll(m)) base = atomicAdd(&list_count[p], (uint32_t)__popcll(m));
+    base = __shfl(base, __builtin_ctzll(m), 64);
+    if (nonempty && (uint32_t)(r & 1) == p)
+      region_list[p * list_stride + base + (uint32_t)__popcll(m & lt)] = (uint32_t)r;
+  }
+}
+
+extern "C" __global__ void gqf_region_lists_reset(uint32_t* __restrict__ list_count) {
+  if (threadIdx.x < 2) list_count[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -875,21 +913,29 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
 // launchers
 
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
-                              const uint64_t* n_dev, uint32_t* region_start, hipStream_t s) {
+                              const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
+                              uint32_t* list_count, hipStream_t s) {
   const uint32_t g = (uint32_t)((nregions + 1 + 255) / 256);
   gqf_prof_begin("region_bounds", s);
-  hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(256), 0, s, qf_dev, items, n, n_dev, region_start);
+  hipLaunchKernelGGL(gqf_region_lists_reset, dim3(1), dim3(64), 0, s, list_count);
+  hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(256), 0, s, qf_dev, items, n, n_dev, region_start,
+                     region_list, list_count, (uint64_t)((nregions + 1) / 2 + 1));
   gqf_prof_end("region_bounds", s);
 }
 
 void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* items, const uint64_t* counts,
-                        const uint32_t* region_start, int mode, uint8_t* wide_flags, uint32_t* wide_count,
+                        uint64_t n_items, const uint32_t* region_start, const uint32_t* region_list,
+                        const uint32_t* list_count, int mode, uint8_t* wide_flags, uint32_t* wide_count,
                         hipStream_t s) {
-  const uint32_t g = (uint32_t)((nregions + 1) / 2);
+  // at most one workgroup per non-empty region: no more than the regions
+  // of a parity, nor than the items
+  const uint64_t per_parity = (nregions + 1) / 2;
+  const uint32_t g = (uint32_t)(n_items < per_parity ? (n_items ? n_items : 1) : per_parity);
+  const uint64_t stride = per_parity + 1;
   for (int parity = 0; parity < 2; parity++) {
     gqf_prof_begin("rebuild_phase", s);
-    hipLaunchKernelGGL(gqf_rebuild_phase, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start, parity,
-                       mode, wide_flags, wide_count);
+    hipLaunchKernelGGL(gqf_rebuild_phase, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start,
+                       region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
     gqf_prof_end("rebuild_phase", s);
     gqf_prof_begin("rebuild_wide", s);
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(1), dim3(64), 0, s, qf_dev, items, counts, region_start, parity, mode,

@@ -140,3 +140,32 @@ def kmer_workload(nfilters=256, seed=5, device="cpu", **kw):
         parts.append(kmers(n, glen, seed * 1000003 + f, device=device))
         offsets.append(offsets[-1] + n)
     return params, torch.cat(parts), offsets
+
+
+def serve_workload(n_requests, nfilters=64, seed=6, q_lo=22, q_hi=30, median_log2=14, sigma=1.0,
+                   max_log2=22, rate_per_s=1.0e5):
+    """A served request stream.  Filter f has 2^(q_lo + a_f % (q_hi - q_lo + 1))
+    slots; request i picks a filter uniformly, an operation 60% get / 30%
+    insert / 10% delete, a size lognormal(ln 2^median_log2, sigma) clipped
+    to [1, 2^max_log2], and arrives after an exponential gap of mean
+    1/rate_per_s.  All draws come from splitmix64(seed) (filter sizes from
+    outputs [0, nfilters), request draws from [2^32, ...)); request i's keys
+    are outputs [2^40 + offset_i, ...) of the same stream, offsets running.
+    Returns ({fid: log2 slots}, [(t_arrival, fid, op, n_keys, key_skip)])."""
+    import math
+    a = splitmix64(seed, nfilters).tolist()
+    nbits = {f: q_lo + (a[f] % (1 << 64)) % (q_hi - q_lo + 1) for f in range(nfilters)}
+    u = splitmix64(seed, 5 * n_requests, skip=1 << 32).tolist()
+    unit = lambda x: ((x % (1 << 64)) >> 11) * (2.0 ** -53) + 2.0 ** -54
+    reqs, t, off = [], 0.0, 0
+    for i in range(n_requests):
+        u0, u1, u2, u3, u4 = (unit(x) for x in u[5 * i:5 * i + 5])
+        t += -math.log(u0) / rate_per_s
+        fid = min(nfilters - 1, int(u1 * nfilters))
+        op = "get" if u2 < 0.6 else ("insert" if u2 < 0.9 else "delete")
+        z = math.sqrt(-2.0 * math.log(u3)) * math.cos(2.0 * math.pi * u4)
+        n = int(round(math.exp(median_log2 * math.log(2.0) + sigma * z)))
+        n = max(1, min(1 << max_log2, n))
+        reqs.append((t, fid, op, n, (1 << 40) + off))
+        off += n
+    return nbits, reqs
