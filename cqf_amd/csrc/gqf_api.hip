@@ -6,6 +6,7 @@
 // the filter's host-side state: sizes, the workspace of the bulk pipeline
 // and the device status word.
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -97,15 +98,13 @@ void free_workspace(Workspace& w) {
   w.cap = 0;
 }
 
-// grow the per-key buffers to hold n keys (synchronises when it allocates)
-void ensure_workspace(Filter* f, uint64_t n) {
-  std::lock_guard<std::mutex> lk(f->mu);
-  Workspace& w = f->ws;
+// the per-table part of a workspace: region tables of the rebuild
+void init_region_workspace(Workspace& w, uint64_t nblocks) {
   if (!w.totals) {
     dev_alloc(&w.totals, 1024);
     dev_alloc(&w.m_dev, 1);
     // sized for the smallest tunable region, so retuning never reallocates
-    const uint64_t maxreg = (f->host.nblocks + GQF_REGION_BLOCKS_MIN - 1) / GQF_REGION_BLOCKS_MIN;
+    const uint64_t maxreg = (nblocks + GQF_REGION_BLOCKS_MIN - 1) / GQF_REGION_BLOCKS_MIN;
     dev_alloc(&w.region_start, maxreg + 2);
     dev_alloc(&w.region_list, 2 * ((maxreg + 1) / 2 + 1));
     dev_alloc(&w.list_count, 2);
@@ -115,6 +114,13 @@ void ensure_workspace(Filter* f, uint64_t n) {
     GQF_CHECK(hipMemset(w.wide_count, 0, sizeof(uint32_t)));
     dev_alloc(&w.misses, 1);
   }
+}
+
+// grow the per-key buffers to hold n keys (synchronises when it allocates)
+void ensure_workspace(Filter* f, uint64_t n) {
+  std::lock_guard<std::mutex> lk(f->mu);
+  Workspace& w = f->ws;
+  init_region_workspace(w, f->host.nblocks);
   if (n <= w.cap) return;
   GQF_CHECK(hipDeviceSynchronize());
   free_workspace(w);
@@ -135,8 +141,10 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   ensure_workspace(f, n);
   Workspace& w = f->ws;
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
-  const int bits = hash_it ? (int)f->host.key_bits : 64;
-  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : 0, f->host.range - 1, bits, f->sort_digit, w.hist, w.totals, s);
+  const int vb = (int)f->host.value_bits;
+  const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
+  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
+                  w.hist, w.totals, s);
   const uint64_t* items = keys;
   const uint64_t* counts = nullptr;
   const uint64_t* n_dev = nullptr;
@@ -221,48 +229,87 @@ bool image_to_device_layout(const uint8_t* img, uint64_t nblocks, std::vector<ui
 // ---------------------------------------------------------------------------
 // lifecycle
 
-extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
-  *qf = nullptr;
-  if (nbits < 6 || nbits > 40) {
-    fprintf(stderr, "libgqf: nbits %d out of range\n", nbits);
-    return;
-  }
-  Filter* f = new Filter();
-  QF& h = f->host;
+namespace {
+
+// Sizes of a table of 2^nbits slots and `vb` value bits, and its device
+// storage (blocks zeroed, point locks); status and descriptor are the
+// caller's.  Sizing arithmetic of qf_init / qf_malloc_device
+// (src/gqf.cu:1939-1996, 2128-2140): 8-bit slots, key_bits = nbits + 8 - vb,
+// double-precision sqrt.
+bool alloc_table(QF& h, int nbits, int vb) {
   memset(&h, 0, sizeof(h));
-  // sizing arithmetic of qf_init / qf_malloc_device (src/gqf.cu:1939-1996,
-  // 2128-2140): r = 8 remainder bits, 0 value bits, double-precision sqrt
   h.nslots = 1ULL << nbits;
   h.xnslots = h.nslots + (uint64_t)(10 * sqrt((double)h.nslots));
   h.nblocks = (h.xnslots + GQF_SLOTS_PER_BLOCK - 1) / GQF_SLOTS_PER_BLOCK;
-  h.key_bits = (uint64_t)nbits + 8;
-  h.key_remainder_bits = 8;
+  h.value_bits = (uint64_t)vb;
+  h.key_bits = (uint64_t)nbits + 8 - h.value_bits;
+  h.key_remainder_bits = 8 - h.value_bits;
   h.bits_per_slot = 8;
-  h.range = h.nslots << h.key_remainder_bits;
+  h.range = h.nslots << 8;
   h.num_locks = h.xnslots / GQF_REF_LOCK_SLOTS + 2;
   h.region_blocks = GQF_REGION_BLOCKS;
   h.nregions = (h.nblocks + h.region_blocks - 1) / h.region_blocks;
   const size_t bytes = gqf_table_bytes(h.nblocks);
   dev_alloc(&h.blocks, bytes);
+  if (!h.blocks) return false;
   GQF_CHECK(hipMemset(h.blocks, 0, bytes));  // qf_malloc memsets the image (src/gqf.cu:2081)
-  dev_alloc(&h.status, 1);
-  GQF_CHECK(hipMemset(h.status, 0, sizeof(uint32_t)));
   // the reference allocates the point-API locks only when !bulk_config
   // (src/gqf.cu:2166-2190); they are always allocated here (8 MB at 2^28
   // slots) so a point call on a bulk-configured filter cannot fault
-  (void)bulk_config;
-  {
-    const size_t nl = (size_t)h.num_locks * GQF_REF_LOCK_DIST;
-    dev_alloc(&h.locks, nl);
-    GQF_CHECK(hipMemset(h.locks, 0, nl * sizeof(uint32_t)));
+  const size_t nl = (size_t)h.num_locks * GQF_REF_LOCK_DIST;
+  dev_alloc(&h.locks, nl);
+  if (!h.locks) {
+    dev_free(h.blocks);
+    return false;
   }
+  GQF_CHECK(hipMemset(h.locks, 0, nl * sizeof(uint32_t)));
+  return true;
+}
+
+Filter* create_filter(int nbits, int vb) {
+  if (nbits < 6 || nbits > 40 || vb < 0 || vb > 6) {
+    fprintf(stderr, "libgqf: nbits %d / value bits %d out of range\n", nbits, vb);
+    return nullptr;
+  }
+  Filter* f = new Filter();
+  if (!alloc_table(f->host, nbits, vb)) {
+    delete f;
+    return nullptr;
+  }
+  dev_alloc(&f->host.status, 1);
+  GQF_CHECK(hipMemset(f->host.status, 0, sizeof(uint32_t)));
   dev_alloc(&f->dev, 1);
-  GQF_CHECK(hipMemcpy(f->dev, &h, sizeof(QF), hipMemcpyHostToDevice));
+  GQF_CHECK(hipMemcpy(f->dev, &f->host, sizeof(QF), hipMemcpyHostToDevice));
   {
     std::lock_guard<std::mutex> lk(g_mu);
     g_filters[f->dev] = f;
   }
-  *qf = f->dev;
+  return f;
+}
+
+void free_all_workspace(Workspace& w) {
+  free_workspace(w);
+  dev_free(w.totals);
+  dev_free(w.m_dev);
+  dev_free(w.region_start);
+  dev_free(w.region_list);
+  dev_free(w.list_count);
+  dev_free(w.wide_flags);
+  dev_free(w.wide_count);
+  dev_free(w.misses);
+}
+
+}  // namespace
+
+extern "C" void qf_malloc_device(QF** qf, int nbits, bool bulk_config) {
+  (void)bulk_config;  // locks are always allocated, see alloc_table
+  Filter* f = create_filter(nbits, 0);
+  *qf = f ? f->dev : nullptr;
+}
+
+extern "C" void qf_malloc_device_kv(QF** qf, int nbits, int value_bits) {
+  Filter* f = create_filter(nbits, value_bits);
+  *qf = f ? f->dev : nullptr;
 }
 
 extern "C" void qf_destroy_device(QF* qf) {
@@ -275,15 +322,7 @@ extern "C" void qf_destroy_device(QF* qf) {
     g_filters.erase(it);
   }
   GQF_CHECK(hipDeviceSynchronize());
-  free_workspace(f->ws);
-  dev_free(f->ws.totals);
-  dev_free(f->ws.m_dev);
-  dev_free(f->ws.region_start);
-  dev_free(f->ws.region_list);
-  dev_free(f->ws.list_count);
-  dev_free(f->ws.wide_flags);
-  dev_free(f->ws.wide_count);
-  dev_free(f->ws.misses);
+  free_all_workspace(f->ws);
   dev_free(f->host.blocks);
   dev_free(f->host.status);
   dev_free(f->host.locks);
@@ -387,11 +426,12 @@ extern "C" int qf_get_info(QF* qf, gqf_info* out) {
   out->key_bits = h.key_bits;
   out->key_remainder_bits = h.key_remainder_bits;
   out->bits_per_slot = h.bits_per_slot;
-  out->range = h.range;
+  out->range = 1ULL << h.key_bits;  // the reference's range: nslots << key_remainder_bits
   out->num_locks = h.num_locks;
   out->nregions = h.nregions;
   out->region_blocks = h.region_blocks;
   out->sort_digit_bits = (uint64_t)f->sort_digit;
+  out->value_bits = h.value_bits;
   out->total_size_in_bytes = h.nblocks * GQF_REF_BLOCK_BYTES;  // the reference's image size
   out->device_bytes = gqf_table_bytes(h.nblocks);
   return 0;
@@ -622,4 +662,327 @@ extern "C" int gqf_profile_read(const char* name, double* total_ms, uint64_t* la
 extern "C" void gqf_fill_splitmix64(uint64_t* dev_out, uint64_t n, uint64_t seed, uint64_t skip, void* stream) {
   gqf_launch_splitmix64(dev_out, n, seed, skip, (hipStream_t)stream);
   GQF_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// whole-table operations: enumeration, statistics, census, files, resize,
+// merge
+
+namespace {
+
+// device scratch of one count pass over nblocks blocks
+struct EnumScratch {
+  uint64_t nwg = 0;
+  uint32_t* wg_items = nullptr;
+  uint64_t* wg_base = nullptr;           // nwg + 1
+  unsigned long long* stats = nullptr;   // items, sum of counts, used slots, spare
+  bool init(uint64_t nblocks) {
+    nwg = gqf_enum_wg_count(nblocks);
+    dev_alloc(&wg_items, nwg);
+    dev_alloc(&wg_base, nwg + 1);
+    dev_alloc(&stats, 4);
+    return wg_items && wg_base && stats;
+  }
+  ~EnumScratch() {
+    dev_free(wg_items);
+    dev_free(wg_base);
+    dev_free(stats);
+  }
+};
+
+uint64_t quotient_blocks(const QF& h) { return (h.nslots + GQF_SLOTS_PER_BLOCK - 1) / GQF_SLOTS_PER_BLOCK; }
+
+// Counted insert of every item of `src` into `dst` (dst_dev / dst, with the
+// region tables of dst_ws): the items are enumerated in chunks of whole
+// 256-block groups, each chunk written as ascending unique slot hashes of
+// dst with their counts and merged into dst by the rebuild.  Returns the
+// number of items moved, or < 0.  Synchronises.
+int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF& dst, Workspace& dst_ws,
+                   hipStream_t s) {
+  const uint64_t nb = quotient_blocks(src);
+  EnumScratch sc;
+  if (!sc.init(nb)) return -3;
+  GQF_CHECK(hipMemsetAsync(sc.stats, 0, 4 * sizeof(unsigned long long), s));
+  gqf_launch_enum_count(src_dev, 0, nb, 0, src.nslots, sc.wg_items, sc.wg_base, sc.stats, s);
+  std::vector<uint64_t> base(sc.nwg + 1);
+  GQF_CHECK(hipMemcpyAsync(base.data(), sc.wg_base, base.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  GQF_CHECK(hipStreamSynchronize(s));
+  const uint64_t total = base[sc.nwg];
+  if (total == 0) return 0;
+  uint64_t cap = total < (1ULL << 24) ? total : (1ULL << 24);
+  for (uint64_t u = 0; u < sc.nwg; u++)
+    if (base[u + 1] - base[u] > cap) cap = base[u + 1] - base[u];
+  uint64_t* items = nullptr;
+  uint64_t* counts = nullptr;
+  dev_alloc(&items, cap);
+  dev_alloc(&counts, cap);
+  if (!items || !counts) {
+    dev_free(items);
+    dev_free(counts);
+    return -3;
+  }
+  init_region_workspace(dst_ws, dst.nblocks);
+  for (uint64_t u = 0; u < sc.nwg;) {
+    uint64_t v = u + 1;
+    while (v < sc.nwg && base[v + 1] - base[u] <= cap) v++;
+    const uint64_t n = base[v] - base[u];
+    if (n) {
+      const uint64_t b_lo = u * GQF_ENUM_WG_BLOCKS, b_hi = v * GQF_ENUM_WG_BLOCKS < nb ? v * GQF_ENUM_WG_BLOCKS : nb;
+      const uint64_t q_hi = b_hi * 64 < src.nslots ? b_hi * 64 : src.nslots;
+      gqf_launch_enum_write(src_dev, b_lo, b_hi, b_lo * 64, q_hi, sc.wg_base + u, base[u], GQF_ENUM_OUT_MOVE,
+                            (int)src.value_bits, (int)dst.value_bits, gqf_mask(src.key_bits), items, counts, nullptr,
+                            cap, s);
+      gqf_launch_region_bounds(dst_dev, dst.nregions, items, n, nullptr, dst_ws.region_start, dst_ws.region_list,
+                               dst_ws.list_count, s);
+      gqf_launch_rebuild(dst_dev, dst.nregions, items, counts, n, dst_ws.region_start, dst_ws.region_list,
+                         dst_ws.list_count, kModeInsert, dst_ws.wide_flags, dst_ws.wide_count, s);
+    }
+    u = v;
+  }
+  GQF_CHECK(hipStreamSynchronize(s));
+  dev_free(items);
+  dev_free(counts);
+  return (int64_t)total;
+}
+
+int log2_exact(uint64_t v) {
+  int k = 0;
+  while ((1ULL << k) < v) k++;
+  return (1ULL << k) == v ? k : -1;
+}
+
+// the reference's qfmetadata (include/gqf_int.cuh:96-113) as laid out by
+// an x86-64 compiler: the enum is an int, 4 bytes of padding after `seed`
+struct qf_file_meta {
+  uint64_t magic_endian_number;
+  int32_t hash_mode;
+  uint32_t reserved;
+  uint64_t total_size_in_bytes;
+  uint32_t seed;
+  uint32_t pad;
+  uint64_t nslots, xnslots, key_bits, value_bits, key_remainder_bits, bits_per_slot, range, nblocks;
+  uint64_t nelts, ndistinct_elts, noccupied_slots;
+};
+static_assert(sizeof(qf_file_meta) == 120, "qfmetadata is 120 bytes");
+static_assert(offsetof(qf_file_meta, nslots) == 32, "qfmetadata field offsets");
+constexpr uint64_t kQfMagic = 1018874902021329732ULL;  // MAGIC_NUMBER, include/gqf_int.cuh:23
+
+}  // namespace
+
+extern "C" int qf_enumerate(QF* qf, uint64_t q_lo, uint64_t q_hi, int form, uint64_t* hashes, uint64_t* counts,
+                            uint64_t* values, uint64_t cap, uint64_t* n_items, void* stream) {
+  if (n_items) *n_items = 0;
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  if (form != QF_ENUM_HASH && form != QF_ENUM_KEY) return -2;
+  const QF& h = f->host;
+  hipStream_t s = (hipStream_t)stream;
+  if (q_hi > h.nslots) q_hi = h.nslots;
+  if (q_lo >= q_hi) return 0;
+  const uint64_t b_lo = q_lo / 64, b_hi = (q_hi + 63) / 64;
+  EnumScratch sc;
+  if (!sc.init(b_hi - b_lo)) return -3;
+  GQF_CHECK(hipMemsetAsync(sc.stats, 0, 4 * sizeof(unsigned long long), s));
+  gqf_launch_enum_count(f->dev, b_lo, b_hi, q_lo, q_hi, sc.wg_items, sc.wg_base, sc.stats, s);
+  if (cap && (hashes || counts || values))
+    gqf_launch_enum_write(f->dev, b_lo, b_hi, q_lo, q_hi, sc.wg_base, 0,
+                          form == QF_ENUM_KEY ? GQF_ENUM_OUT_KEY : GQF_ENUM_OUT_HASH, (int)h.value_bits, 0,
+                          gqf_mask(h.key_bits), hashes, counts, values, cap, s);
+  uint64_t total = 0;
+  hipError_t e = hipMemcpyAsync(&total, sc.wg_base + sc.nwg, sizeof(total), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  note_error(e, "qf_enumerate");
+  if (e != hipSuccess) return -3;
+  if (n_items) *n_items = total;
+  return 0;
+}
+
+extern "C" int qf_get_stats(QF* qf, uint64_t* nelts, uint64_t* ndistinct_elts, uint64_t* noccupied_slots) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const uint64_t nb = quotient_blocks(f->host);
+  EnumScratch sc;
+  if (!sc.init(nb)) return -3;
+  GQF_CHECK(hipDeviceSynchronize());
+  GQF_CHECK(hipMemset(sc.stats, 0, 4 * sizeof(unsigned long long)));
+  gqf_launch_enum_count(f->dev, 0, nb, 0, f->host.nslots, sc.wg_items, sc.wg_base, sc.stats, nullptr);
+  unsigned long long st[4] = {0, 0, 0, 0};
+  hipError_t e = hipMemcpy(st, sc.stats, sizeof(st), hipMemcpyDeviceToHost);
+  note_error(e, "qf_get_stats");
+  if (e != hipSuccess) return -3;
+  if (ndistinct_elts) *ndistinct_elts = st[0];
+  if (nelts) *nelts = st[1];
+  if (noccupied_slots) *noccupied_slots = st[2];
+  return 0;
+}
+
+extern "C" int qf_cluster_census(QF* qf, uint64_t* lengths, uint64_t cap, uint64_t* hist, uint32_t nbins,
+                                 uint64_t* num_clusters, uint64_t* max_len) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  if (hist && (nbins == 0 || nbins > 4096)) return -2;
+  const uint64_t nb = f->host.nblocks;
+  const uint64_t nwg = gqf_enum_wg_count(nb);
+  uint64_t* used = nullptr;
+  uint32_t* wg_starts = nullptr;
+  uint64_t* wg_base = nullptr;
+  unsigned long long* mx = nullptr;
+  dev_alloc(&used, nb);
+  dev_alloc(&wg_starts, nwg);
+  dev_alloc(&wg_base, nwg + 1);
+  dev_alloc(&mx, 1);
+  int rc = -3;
+  if (used && wg_starts && wg_base && mx) {
+    GQF_CHECK(hipDeviceSynchronize());
+    GQF_CHECK(hipMemset(mx, 0, sizeof(unsigned long long)));
+    gqf_launch_census(f->dev, nb, used, wg_starts, wg_base, lengths, lengths ? cap : 0,
+                      (unsigned long long*)hist, hist ? nbins : 0, mx, nullptr);
+    uint64_t nc = 0;
+    unsigned long long m = 0;
+    hipError_t e = hipMemcpy(&nc, wg_base + nwg, sizeof(nc), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&m, mx, sizeof(m), hipMemcpyDeviceToHost);
+    note_error(e, "qf_cluster_census");
+    if (e == hipSuccess) {
+      if (num_clusters) *num_clusters = nc;
+      if (max_len) *max_len = (uint64_t)m;
+      rc = 0;
+    }
+  }
+  dev_free(used);
+  dev_free(wg_starts);
+  dev_free(wg_base);
+  dev_free(mx);
+  return rc;
+}
+
+extern "C" uint64_t qf_serialize(QF* qf, const char* filename) {
+  Filter* f = lookup(qf);
+  if (!f || !filename) return 0;
+  const QF& h = f->host;
+  qf_file_meta m;
+  memset(&m, 0, sizeof(m));
+  m.magic_endian_number = kQfMagic;
+  m.hash_mode = QF_HASH_INVERTIBLE;
+  m.total_size_in_bytes = h.nblocks * GQF_REF_BLOCK_BYTES;
+  m.seed = 0;
+  m.nslots = h.nslots;
+  m.xnslots = h.xnslots;
+  m.key_bits = h.key_bits;
+  m.value_bits = h.value_bits;
+  m.key_remainder_bits = h.key_remainder_bits;
+  m.bits_per_slot = h.bits_per_slot;
+  m.range = 1ULL << h.key_bits;
+  m.nblocks = h.nblocks;
+  if (qf_get_stats(qf, &m.nelts, &m.ndistinct_elts, &m.noccupied_slots) != 0) return 0;
+  std::vector<uint8_t> img(m.total_size_in_bytes);
+  if (qf_export_image(qf, img.data(), img.size()) != 0) return 0;
+  FILE* fp = fopen(filename, "wb");
+  if (!fp) {
+    fprintf(stderr, "libgqf: qf_serialize: cannot open %s\n", filename);
+    return 0;
+  }
+  bool ok = fwrite(&m, sizeof(m), 1, fp) == 1 && fwrite(img.data(), 1, img.size(), fp) == img.size();
+  ok = (fclose(fp) == 0) && ok;
+  return ok ? sizeof(m) + img.size() : 0;
+}
+
+extern "C" uint64_t qf_deserialize_device(QF** qf, const char* filename) {
+  *qf = nullptr;
+  if (!filename) return 0;
+  FILE* fp = fopen(filename, "rb");
+  if (!fp) {
+    fprintf(stderr, "libgqf: qf_deserialize_device: cannot open %s\n", filename);
+    return 0;
+  }
+  qf_file_meta m;
+  bool ok = fread(&m, sizeof(m), 1, fp) == 1;
+  const int nbits = ok ? log2_exact(m.nslots) : -1;
+  ok = ok && m.magic_endian_number == kQfMagic && m.hash_mode == QF_HASH_INVERTIBLE && nbits >= 6 && nbits <= 40 &&
+       m.bits_per_slot == 8 && m.value_bits <= 6 && m.key_remainder_bits + m.value_bits == 8 &&
+       m.key_bits == (uint64_t)nbits + m.key_remainder_bits &&
+       m.range == (m.nslots << m.key_remainder_bits) &&
+       m.xnslots == m.nslots + (uint64_t)(10 * sqrt((double)m.nslots)) && m.nblocks == (m.xnslots + 63) / 64 &&
+       m.total_size_in_bytes == m.nblocks * GQF_REF_BLOCK_BYTES;
+  std::vector<uint8_t> img;
+  if (ok) {
+    img.resize(m.total_size_in_bytes);
+    ok = fread(img.data(), 1, img.size(), fp) == img.size();
+  }
+  fclose(fp);
+  if (!ok) {
+    fprintf(stderr, "libgqf: qf_deserialize_device: %s is not a CQF image this library reads\n", filename);
+    return 0;
+  }
+  Filter* f = create_filter(nbits, (int)m.value_bits);
+  if (!f) return 0;
+  if (qf_import_image(f->dev, img.data(), img.size()) != 0) {
+    qf_destroy_device(f->dev);
+    return 0;
+  }
+  *qf = f->dev;
+  return sizeof(m) + img.size();
+}
+
+extern "C" int64_t qf_resize_device(QF* qf, int new_nbits) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  const QF old = f->host;
+  const int nbits = log2_exact(old.nslots);
+  if (new_nbits < nbits || new_nbits > 40) return -2;
+  const int vb = (int)old.value_bits + (new_nbits - nbits);
+  if (vb > 6) return -2;
+  GQF_CHECK(hipDeviceSynchronize());
+  QF nh;
+  if (!alloc_table(nh, new_nbits, vb)) return -3;
+  nh.status = old.status;
+  QF* ndev = nullptr;
+  dev_alloc(&ndev, 1);
+  Workspace nws;
+  int64_t moved = -3;
+  if (ndev && hipMemcpy(ndev, &nh, sizeof(QF), hipMemcpyHostToDevice) == hipSuccess)
+    moved = move_items(f->dev, old, ndev, nh, nws, nullptr);
+  if (moved < 0) {
+    free_all_workspace(nws);
+    dev_free(nh.blocks);
+    dev_free(nh.locks);
+    dev_free(ndev);
+    return moved;
+  }
+  // the handle's descriptor now describes the new table
+  std::lock_guard<std::mutex> lk(f->mu);
+  free_all_workspace(f->ws);
+  f->ws = nws;
+  QF gone = f->host;
+  f->host = nh;
+  GQF_CHECK(hipMemcpy(f->dev, &f->host, sizeof(QF), hipMemcpyHostToDevice));
+  dev_free(gone.blocks);
+  dev_free(gone.locks);
+  dev_free(ndev);
+  return moved;
+}
+
+extern "C" int64_t qf_multi_merge_device(QF* const* qfs, int nqf, QF* qfr) {
+  Filter* r = lookup(qfr);
+  if (!r || nqf < 0 || (nqf && !qfs)) return -1;
+  std::vector<Filter*> src;
+  for (int i = 0; i < nqf; i++) {
+    Filter* f = lookup(qfs[i]);
+    if (!f) return -1;
+    if (f == r || f->host.key_bits != r->host.key_bits || f->host.value_bits > r->host.value_bits) return -2;
+    src.push_back(f);
+  }
+  GQF_CHECK(hipDeviceSynchronize());
+  int64_t total = 0;
+  for (Filter* f : src) {
+    std::lock_guard<std::mutex> lk(r->mu);
+    const int64_t m = move_items(f->dev, f->host, r->dev, r->host, r->ws, nullptr);
+    if (m < 0) return m;
+    total += m;
+  }
+  return total;
+}
+
+extern "C" int64_t qf_merge_device(QF* qfa, QF* qfb, QF* qfc) {
+  QF* const two[2] = {qfa, qfb};
+  return qf_multi_merge_device(two, 2, qfc);
 }

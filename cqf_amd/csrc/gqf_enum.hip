@@ -1,0 +1,361 @@
+// gqf_enum.hip -- whole-table passes over a filter: enumeration of its items
+// in ascending hash order, the item / count / used-slot statistics, and the
+// cluster census.
+//
+// Enumeration replaces the reference's host iterator
+// (qf_iterator_from_position src/gqf.cu:3655-3700, qfi_next :3814-3840,
+// qfi_get_hash :3807, qfi_get_key :3791-3805).  Table order is hash order:
+// runs sit in quotient order and a run's counters in remainder order, so
+// the iterator's sequence is the table's counters read left to right.  Here
+// one thread owns one 64-slot block and decodes the runs of the block's
+// occupied quotients (which may start or end in later blocks); a count
+// pass, a scan of per-workgroup totals and a write pass put every item at
+// its rank.
+//
+// The census replaces src/cluster_length_test.cu:37-120, which walks the
+// table with find_first_empty_slot (src/gqf.cu:728-771) from one thread:
+// a cluster is a maximal stretch of used slots.  Here each block's used
+// slots become a 64-bit mask; cluster starts are the mask's rising edges,
+// and each start finds its cluster's end by scanning the masks forward.
+#include "gqf_internal.h"
+#include "gqf_launch.h"
+
+namespace {
+
+constexpr int kThreads = GQF_ENUM_WG_BLOCKS;  // one thread per table block
+
+// Slot reader for the runs of block b: slots of blocks b and b + 1 come
+// from a per-thread LDS row (loaded with 16-byte vector loads), later ones
+// from the table.
+struct slot_row {
+  const QF* f;
+  uint64_t base;        // first slot of block b
+  const uint8_t* row;   // 128 bytes in LDS
+  __device__ uint8_t operator()(uint64_t s) const {
+    const uint64_t rel = s - base;
+    return rel < 128 ? row[rel] : gqf_point::slot(*f, s);
+  }
+};
+
+// slots of blocks b and b + 1 into a 33-word LDS row (the odd stride keeps
+// the rows of a wave in distinct banks)
+__device__ inline void load_row(const QF& f, uint64_t b, uint32_t* row) {
+  const GQF_GLOBAL uint4* p = gqf_block_ptr(f, b);
+  const bool next = b + 1 < f.nblocks;
+  const GQF_GLOBAL uint4* p2 = gqf_block_ptr(f, next ? b + 1 : b);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    uint4 v = k < 4 ? p[4 + k] : p2[k];
+    if (k >= 4 && !next) v = make_uint4(0, 0, 0, 0);
+    row[4 * k] = v.x;
+    row[4 * k + 1] = v.y;
+    row[4 * k + 2] = v.z;
+    row[4 * k + 3] = v.w;
+  }
+}
+
+// Visits the counters of the runs of block b's occupied quotients in
+// [q_lo, q_hi), in table order: item(storage hash, count).  Returns the
+// slots those runs use, or -1 if a run has no run end (a damaged table).
+template <typename Item>
+__device__ inline int64_t walk_items(const QF& f, uint64_t b, uint64_t q_lo, uint64_t q_hi, const slot_row& sl,
+                                     Item item) {
+  uint64_t occ = gqf_point::occ_word(f, b);
+  int64_t prev = gqf_point::end_before_block(f, b);
+  int64_t used = 0;
+  while (occ) {
+    const uint64_t q = b * 64 + (uint64_t)__builtin_ctzll(occ);
+    occ &= occ - 1;
+    const uint64_t start = (int64_t)q > prev + 1 ? q : (uint64_t)(prev + 1);
+    const int64_t e = gqf_point::runend_from(f, start, 0);
+    if (e < (int64_t)start) return -1;
+    prev = e;
+    if (q < q_lo || q >= q_hi) continue;
+    const uint32_t len = (uint32_t)(e - (int64_t)start + 1);
+    used += len;
+    auto at = [&](uint32_t k) -> uint64_t { return sl(start + k); };
+    uint32_t idx = 0;
+    while (idx < len) {
+      uint64_t rem, cnt;
+      const uint32_t last = gqf_decode_counter(at, len, idx, &rem, &cnt);
+      item((q << 8) | rem, cnt);
+      idx = last + 1;
+    }
+  }
+  return used;
+}
+
+// workgroup exclusive scan of one u32 per thread; returns the prefix and
+// writes the workgroup total to *total
+__device__ inline uint32_t wg_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; k++) {
+    if (k < w) before += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+__device__ inline uint64_t wg_sum_u64(uint64_t v) {
+  __shared__ uint64_t wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (lane == 0) wsum[w] = v;
+  __syncthreads();
+  uint64_t t = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; k++) t += wsum[k];
+  __syncthreads();
+  return t;
+}
+
+// Output form of an item (storage hash S = (hash << value_bits) | value):
+//   GQF_ENUM_OUT_HASH  S >> value_bits (qfi_get_hash's key), value to `values`
+//   GQF_ENUM_OUT_KEY   hash_64i of that (qfi_get_key's key)
+//   GQF_ENUM_OUT_MOVE  the storage hash of the same (hash, value) in a filter
+//                      with `dst_vb` value bits (value truncated to them)
+struct out_form {
+  int mode;
+  int vb;       // source value bits
+  int dst_vb;   // MOVE: destination value bits
+  uint64_t key_mask;
+};
+
+__device__ inline uint64_t form_hash(const out_form& o, uint64_t s) {
+  const uint64_t h = s >> o.vb;
+  if (o.mode == GQF_ENUM_OUT_KEY) return gqf_hash64_inv(h, o.key_mask);
+  if (o.mode == GQF_ENUM_OUT_MOVE) return (h << o.dst_vb) | (s & gqf_mask((uint64_t)min(o.vb, o.dst_vb)));
+  return h;
+}
+
+// Pass 1: items per workgroup; statistics of the whole range into stats[0..2]
+// (items, sum of counts, used slots) and damaged-table flag into status.
+__global__ void __launch_bounds__(kThreads)
+gqf_enum_count(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+               uint32_t* __restrict__ wg_items, unsigned long long* __restrict__ stats) {
+  __shared__ uint32_t rows[kThreads * 33];
+  const QF& f = *qfp;
+  const uint64_t b = b_lo + (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint32_t items = 0;
+  uint64_t sum = 0, used = 0;
+  if (b < b_hi) {
+    uint32_t* row = rows + threadIdx.x * 33;
+    load_row(f, b, row);
+    const slot_row sl{&f, b * 64, (const uint8_t*)row};
+    const int64_t u = walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t, uint64_t c) {
+      items++;
+      sum += c;
+    });
+    if (u < 0) atomicOr(f.status, GQF_STATUS_CLUSTER_OVERFLOW);
+    else used = (uint64_t)u;
+  }
+  uint32_t tot;
+  (void)wg_exclusive_scan(items, &tot);
+  const uint64_t s_sum = wg_sum_u64(sum);
+  const uint64_t s_used = wg_sum_u64(used);
+  if (threadIdx.x == 0) {
+    wg_items[blockIdx.x] = tot;
+    if (tot) atomicAdd(&stats[0], (unsigned long long)tot);
+    if (s_sum) atomicAdd(&stats[1], (unsigned long long)s_sum);
+    if (s_used) atomicAdd(&stats[2], (unsigned long long)s_used);
+  }
+}
+
+// Exclusive scan of n u32 counts into u64 bases (one workgroup); the total
+// goes to base[n].
+__global__ void __launch_bounds__(1024) gqf_enum_scan(const uint32_t* __restrict__ cnt, uint32_t n,
+                                                      uint64_t* __restrict__ base) {
+  __shared__ uint64_t wsum[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t carry = 0;
+  for (uint32_t i0 = 0; i0 < n; i0 += 1024) {
+    const uint32_t i = i0 + threadIdx.x;
+    const uint64_t v = i < n ? cnt[i] : 0;
+    uint64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if (k < w) before += wsum[k];
+      tot += wsum[k];
+    }
+    if (i < n) base[i] = carry + before + incl - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) base[n] = carry;
+}
+
+// Pass 2: every item to its rank (entries at ranks >= cap are dropped).
+__global__ void __launch_bounds__(kThreads)
+gqf_enum_write(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+               const uint64_t* __restrict__ wg_base, uint64_t r0, out_form o, uint64_t* __restrict__ hashes,
+               uint64_t* __restrict__ counts, uint64_t* __restrict__ values, uint64_t cap) {
+  __shared__ uint32_t rows[kThreads * 33];
+  const QF& f = *qfp;
+  const uint64_t b = b_lo + (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const bool mine = b < b_hi;
+  uint32_t* row = rows + threadIdx.x * 33;
+  uint32_t items = 0;
+  if (mine) {
+    load_row(f, b, row);
+    const slot_row sl{&f, b * 64, (const uint8_t*)row};
+    walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t, uint64_t) { items++; });
+  }
+  uint32_t tot;
+  uint64_t r = wg_base[blockIdx.x] - r0 + wg_exclusive_scan(items, &tot);
+  if (!mine || items == 0 || r >= cap) return;
+  const slot_row sl{&f, b * 64, (const uint8_t*)row};
+  const uint64_t vmask = gqf_mask((uint64_t)o.vb);
+  walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t s, uint64_t c) {
+    if (r < cap) {
+      if (hashes) hashes[r] = form_hash(o, s);
+      if (counts) counts[r] = c;
+      if (values) values[r] = s & vmask;
+    }
+    r++;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// census
+
+// used-slot mask of block b, slots >= xnslots excluded.  Slot 64 b + i is
+// used iff the runs of earlier blocks reach it (i < offset) or some quotient
+// of block b at or below it has not ended its run before it
+// (offset_lower_bound != 0, src/gqf.cu:681-705).
+__device__ inline uint64_t used_mask(const QF& f, uint64_t b) {
+  const gqf_header h = gqf_load_header(f, b);
+  uint64_t used;
+  if (h.offset >= 64) {
+    used = ~0ULL;
+  } else {
+    const int off = (int)h.offset;
+    used = gqf_mask((uint64_t)off);
+    int p = __builtin_popcountll(h.occupieds & gqf_mask((uint64_t)off));
+    int r = 0;
+    for (int i = off; i < 64; i++) {
+      p += (int)((h.occupieds >> i) & 1);
+      if (p > r) used |= 1ULL << i;
+      r += (int)((h.runends >> i) & 1);
+    }
+  }
+  const uint64_t lim = f.xnslots - b * 64;
+  if (lim < 64) used &= gqf_mask(lim);
+  return used;
+}
+
+// whether slot 64 b - 1 is used: earlier runs reach into block b, or a run
+// ends exactly there
+__device__ inline uint64_t prev_slot_used(const QF& f, uint64_t b) {
+  return b > 0 && (gqf_point::offset(f, b) > 0 || (gqf_load_runends(f, b - 1) >> 63)) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads)
+gqf_census_mask(const QF* __restrict__ qfp, uint64_t* __restrict__ used, uint32_t* __restrict__ wg_starts) {
+  const QF& f = *qfp;
+  const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint32_t starts = 0;
+  if (b < f.nblocks) {
+    const uint64_t u = used_mask(f, b);
+    used[b] = u;
+    const uint64_t prev = prev_slot_used(f, b);
+    starts = (uint32_t)__builtin_popcountll(u & ~((u << 1) | prev));
+  }
+  uint32_t tot;
+  (void)wg_exclusive_scan(starts, &tot);
+  if (threadIdx.x == 0) wg_starts[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kThreads)
+gqf_census_write(const QF* __restrict__ qfp, const uint64_t* __restrict__ used, const uint64_t* __restrict__ wg_base,
+                 uint64_t* __restrict__ lengths, uint64_t cap, unsigned long long* __restrict__ hist, uint32_t nbins,
+                 unsigned long long* __restrict__ max_len) {
+  extern __shared__ uint32_t lhist[];
+  const QF& f = *qfp;
+  const uint64_t nb = f.nblocks;
+  for (uint32_t i = threadIdx.x; i < nbins; i += kThreads) lhist[i] = 0;
+  __syncthreads();
+  const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  uint64_t starts = 0;
+  if (b < nb) {
+    const uint64_t u = used[b];
+    starts = u & ~((u << 1) | prev_slot_used(f, b));
+  }
+  uint32_t tot;
+  uint64_t r = wg_base[blockIdx.x] + wg_exclusive_scan((uint32_t)__builtin_popcountll(starts), &tot);
+  uint64_t my_max = 0;
+  while (starts) {
+    const int i = __builtin_ctzll(starts);
+    starts &= starts - 1;
+    // first unused slot after 64 b + i
+    uint64_t cb = b;
+    uint64_t free_bits = ~used[cb] & (~0ULL << i);
+    while (!free_bits && ++cb < nb) free_bits = ~used[cb];
+    const uint64_t end = cb < nb ? cb * 64 + (uint64_t)__builtin_ctzll(free_bits) : nb * 64;
+    const uint64_t len = end - (b * 64 + (uint64_t)i);
+    if (lengths && r < cap) lengths[r] = len;
+    r++;
+    if (nbins) atomicAdd(&lhist[len < nbins ? len : nbins - 1], 1u);
+    if (len > my_max) my_max = len;
+  }
+  __syncthreads();
+  if (hist)
+    for (uint32_t i = threadIdx.x; i < nbins; i += kThreads)
+      if (lhist[i]) atomicAdd(&hist[i], (unsigned long long)lhist[i]);
+  if (my_max) atomicMax(max_len, (unsigned long long)my_max);
+}
+
+}  // namespace
+
+uint64_t gqf_enum_wg_count(uint64_t nblocks) { return (nblocks + kThreads - 1) / kThreads; }
+
+void gqf_launch_enum_count(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats, hipStream_t s) {
+  const uint64_t nwg = gqf_enum_wg_count(b_hi - b_lo);
+  if (nwg == 0) return;
+  hipLaunchKernelGGL(gqf_enum_count, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, b_lo, b_hi, q_lo, q_hi,
+                     wg_items, stats);
+  hipLaunchKernelGGL(gqf_enum_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)wg_items, (uint32_t)nwg, wg_base);
+}
+
+void gqf_launch_enum_write(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+                           const uint64_t* wg_base, uint64_t r0, int mode, int vb, int dst_vb, uint64_t key_mask,
+                           uint64_t* hashes, uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s) {
+  const uint64_t nwg = gqf_enum_wg_count(b_hi - b_lo);
+  if (nwg == 0 || cap == 0) return;
+  const out_form o{mode, vb, dst_vb, key_mask};
+  hipLaunchKernelGGL(gqf_enum_write, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, b_lo, b_hi, q_lo, q_hi,
+                     wg_base, r0, o, hashes, counts, values, cap);
+}
+
+void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint32_t* wg_starts, uint64_t* wg_base,
+                       uint64_t* lengths, uint64_t cap, unsigned long long* hist, uint32_t nbins,
+                       unsigned long long* max_len, hipStream_t s) {
+  const uint64_t nwg = gqf_enum_wg_count(nblocks);
+  if (nwg == 0) return;
+  hipLaunchKernelGGL(gqf_census_mask, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, used, wg_starts);
+  hipLaunchKernelGGL(gqf_enum_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)wg_starts, (uint32_t)nwg, wg_base);
+  hipLaunchKernelGGL(gqf_census_write, dim3((uint32_t)nwg), dim3(kThreads), nbins * sizeof(uint32_t), s, qf_dev,
+                     (const uint64_t*)used, (const uint64_t*)wg_base, lengths, cap, hist, nbins, max_len);
+}

@@ -66,13 +66,15 @@ extern "C" __global__ void __launch_bounds__(256)
 gqf_group_tag(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n, int is_hash) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t f = seg_of(g.seg, g.nf, i);
-    const uint64_t range = g.descs[f].range;
+    const QF& d = g.descs[f];
     uint64_t h = keys[i];
     if (!is_hash) {
-      h = gqf_hash64(h, range - 1);
-    } else if (h >= range) {
+      h = gqf_key_hash(d, h);
+    } else if (h > gqf_mask(d.key_bits)) {
       h = (1ULL << g.tag_shift) - 1;
-      atomicOr(g.descs[f].status, GQF_STATUS_BAD_HASH);
+      atomicOr(d.status, GQF_STATUS_BAD_HASH);
+    } else {
+      h <<= d.value_bits;
     }
     keys[i] = ((uint64_t)f << g.tag_shift) | h;
   }
@@ -152,7 +154,8 @@ bool refresh(Group* g, const uint64_t* offsets, bool is_hash, gqf_group_dev* dev
     ev[f + 1] = ev[f] + (nr + 1) / 2;
     od[f + 1] = od[f] + nr / 2;
     rb[f + 1] = rb[f] + nr;
-    if (descs[f].key_bits > maxbits) maxbits = descs[f].key_bits;
+    const uint64_t slot_bits = descs[f].key_bits + descs[f].value_bits;  // log2 of the slot-hash range
+    if (slot_bits > maxbits) maxbits = slot_bits;
   }
   if (offsets && offsets[0] != 0) return false;
   const int shift = (int)maxbits + (is_hash ? 1 : 0);
@@ -210,7 +213,7 @@ void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t
   hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
   uint32_t fbits = 0;
   while ((1ULL << fbits) < dev.nf) fbits++;
-  gqf_launch_sort(keys, g->tmp, n, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
+  gqf_launch_sort(keys, g->tmp, n, 0, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
   hipLaunchKernelGGL(gqf_group_reset, dim3(1), dim3(64), 0, s, g->d_counters);
   gqf_launch_group_rebuild(dev, keys, mode, gqf_group_rebuild_grid(), s);
   hipLaunchKernelGGL(gqf_group_untag, dim3(flat_grid(n)), dim3(256), 0, s, keys, n, dev.tag_shift);

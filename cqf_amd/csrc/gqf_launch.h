@@ -5,12 +5,14 @@
 
 #include "gqf_internal.h"
 
-// radix sort of n keys in place (tmp holds n keys); the first pass hashes
-// with `mask` when hash_it is set.  hist must hold gqf_sort_hist_words(n).
+// radix sort of n keys in place (tmp holds n keys); the first pass turns
+// keys into slot hashes: hash_it 1 hashes under `mask` and shifts left by
+// vshift, 2 shifts pre-hashed keys (a key above mask becomes ~0), 0 keeps
+// them.  hist must hold gqf_sort_hist_words(n).
 uint64_t gqf_sort_hist_words(uint64_t n);
 // digit_bits: 6..10 fixes the radix digit (passes rounded up to even), 0 picks 4 passes
-void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits, int digit_bits,
-                     uint32_t* hist, uint32_t* totals, hipStream_t s);
+void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int vshift, int bits,
+                     int digit_bits, uint32_t* hist, uint32_t* totals, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
 // tile_heads: ceil(n/4096) words; upos: n words.
@@ -45,6 +47,34 @@ void gqf_launch_point_query(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_
                             int concurrent, unsigned long long* misses, hipStream_t s);
 
 void gqf_launch_splitmix64(uint64_t* out, uint64_t n, uint64_t seed, uint64_t skip, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Whole-table passes (gqf_enum.hip).  One thread per table block, 256
+// blocks per workgroup.
+
+#define GQF_ENUM_WG_BLOCKS 256  // table blocks per workgroup of these passes
+// output forms of the enumeration write pass
+enum { GQF_ENUM_OUT_HASH = 0, GQF_ENUM_OUT_KEY = 1, GQF_ENUM_OUT_MOVE = 2 };
+uint64_t gqf_enum_wg_count(uint64_t nblocks);
+// count pass over blocks [b_lo, b_hi), items of quotients [q_lo, q_hi):
+// items per workgroup -> wg_items, their exclusive prefix sums -> wg_base
+// (nwg + 1 entries, the last the total); stats[0..2] += items, sum of
+// counts, slots used by the runs
+void gqf_launch_enum_count(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats, hipStream_t s);
+// write pass over the same range: item of rank r (wg_base[wg] - r0 + rank
+// in the workgroup) -> hashes[r] / counts[r] / values[r] when r < cap.
+// MOVE writes the slot hash the item has in a filter with dst_vb value bits.
+void gqf_launch_enum_write(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
+                           const uint64_t* wg_base, uint64_t r0, int mode, int vb, int dst_vb, uint64_t key_mask,
+                           uint64_t* hashes, uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s);
+// cluster census: used-slot masks (nblocks words), cluster starts per
+// workgroup and their bases (nwg + 1, the last = number of clusters);
+// lengths[rank] (rank < cap), hist[min(len, nbins - 1)] += 1 (nbins <= 4096,
+// 0 for none), *max_len = max(*max_len, longest)
+void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint32_t* wg_starts, uint64_t* wg_base,
+                       uint64_t* lengths, uint64_t cap, unsigned long long* hist, uint32_t nbins,
+                       unsigned long long* max_len, hipStream_t s);
 
 // Per-kernel event timing (off by default).  When enabled, the launchers
 // bracket each kernel with hipEvents on its stream; gqf_profile_read sums

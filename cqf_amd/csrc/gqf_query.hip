@@ -94,11 +94,10 @@ gqf_bulk_get_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ val
                     uint64_t* __restrict__ returns) {
   __shared__ uint32_t rows[256 * 17];
   const QF qf = *qfp;
-  const uint64_t mask = qf.range - 1;
   uint32_t* row = rows + 17 * threadIdx.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    returns[i] = gqf_count_hash(qf, gqf_hash64(vals[i], mask), row);
+    returns[i] = gqf_count_hash(qf, gqf_key_hash(qf, vals[i]), row);
   }
 }
 
@@ -108,12 +107,11 @@ gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
                        unsigned long long* __restrict__ misses) {
   __shared__ uint32_t rows[256 * 17];
   const QF qf = *qfp;
-  const uint64_t mask = qf.range - 1;
   uint32_t* row = rows + 17 * threadIdx.x;
   uint32_t mine = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    mine += gqf_count_hash(qf, gqf_hash64(vals[i], mask), row) < 1;
+    mine += gqf_count_hash(qf, gqf_key_hash(qf, vals[i]), row) < 1;
   }
   // wave reduction, one atomic per wave
   for (int d = 32; d >= 1; d >>= 1) mine += __shfl_down(mine, d, 64);
@@ -136,7 +134,7 @@ gqf_group_get_kernel(gqf_group_dev g, const uint64_t* __restrict__ vals, uint64_
     QF q;
     q.blocks = qf.blocks;
     q.nblocks = qf.nblocks;
-    returns[i] = gqf_count_hash(q, gqf_hash64(vals[i], qf.range - 1), row);
+    returns[i] = gqf_count_hash(q, gqf_key_hash(qf, vals[i]), row);
   }
 }
 

@@ -21,17 +21,23 @@ constexpr int kTile = kThreads * kItems;  // 4096 keys
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRadix = 1024;           // digits of up to 10 bits
 
-__device__ inline uint64_t load_key(const uint64_t* in, uint64_t i, uint64_t n, int hash_it, uint64_t mask) {
+// hash_it: 0 keys as they are; 1 hash under `mask`, then shift left by
+// `vshift` (the filter's value bits); 2 pre-hashed keys shifted by vshift,
+// a key above `mask` becoming ~0 (past every region, flagged there).
+__device__ inline uint64_t load_key(const uint64_t* in, uint64_t i, uint64_t n, int hash_it, uint64_t mask,
+                                    int vshift) {
   if (i >= n) return ~0ULL;
   uint64_t k = in[i];
-  return hash_it ? gqf_hash64(k, mask) : k;
+  if (hash_it == 1) return gqf_hash64(k, mask) << vshift;
+  if (hash_it == 2) return (k & ~mask) ? ~0ULL : (k << vshift);
+  return k;
 }
 }  // namespace
 
 // per-tile digit histogram -> hist[d * ntiles + tile]
 extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits, int hash_it,
-                 uint64_t mask, uint32_t* __restrict__ hist, uint32_t ntiles) {
+                 uint64_t mask, int vshift, uint32_t* __restrict__ hist, uint32_t ntiles) {
   __shared__ uint32_t h[kMaxRadix];
   const int radix = 1 << bits;
   for (int d = threadIdx.x; d < radix; d += kThreads) h[d] = 0;
@@ -42,7 +48,7 @@ gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bit
   for (int k = 0; k < kItems; k++) {
     uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
     if (i < n) {
-      uint64_t key = load_key(in, i, n, hash_it, mask);
+      uint64_t key = load_key(in, i, n, hash_it, mask, vshift);
       atomicAdd(&h[(uint32_t)(key >> shift) & (radix - 1)], 1u);
     }
   }
@@ -97,7 +103,7 @@ gqf_sort_scan_totals(uint32_t* __restrict__ totals, int radix) {
 // Stable scatter of one tile.
 extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int shift,
-                   int bits, int hash_it, uint64_t mask, const uint32_t* __restrict__ hist,
+                   int bits, int hash_it, uint64_t mask, int vshift, const uint32_t* __restrict__ hist,
                    const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
   __shared__ uint32_t wcnt[kWaves][kMaxRadix];
   __shared__ uint32_t tile_start[kMaxRadix];
@@ -120,7 +126,7 @@ gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, 
   for (int k = 0; k < kItems; k++) {
     uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
     bool valid = i < n;
-    uint64_t key = load_key(in, i, n, hash_it, mask);
+    uint64_t key = load_key(in, i, n, hash_it, mask, vshift);
     uint32_t d = (uint32_t)(key >> shift) & (radix - 1);
     // lanes holding the same digit (and a valid key)
     uint64_t peers = __ballot(valid);
@@ -275,8 +281,8 @@ static inline uint32_t sort_ntiles(uint64_t n) { return (uint32_t)((n + kTile - 
 
 uint64_t gqf_sort_hist_words(uint64_t n) { return (uint64_t)kMaxRadix * (sort_ntiles(n) ? sort_ntiles(n) : 1); }
 
-void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int bits, int digit_bits,
-                     uint32_t* hist, uint32_t* totals, hipStream_t s) {
+void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int vshift, int bits,
+                     int digit_bits, uint32_t* hist, uint32_t* totals, hipStream_t s) {
   if (n == 0) return;
   // an even number of passes leaves the result in `keys`
   int passes, digit;
@@ -298,14 +304,14 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
   uint64_t* out = tmp;
   for (int p = 0; p < passes; p++) {
     const int shift = p * digit;
-    const int h = (hash_it && p == 0) ? 1 : 0;
+    const int h = p == 0 ? hash_it : 0;
     gqf_prof_begin("sort_pass", s);
-    hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n, shift, digit, h, mask, hist,
-                       ntiles);
+    hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n, shift, digit, h, mask, vshift,
+                       hist, ntiles);
     hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
     hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
     hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)in, out, n, shift,
-                       digit, h, mask, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+                       digit, h, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
     gqf_prof_end("sort_pass", s);
     uint64_t* t = in;
     in = out;

@@ -39,14 +39,19 @@ EXPORTS = [
     "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
+    "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
+    "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
 ]
+
+QF_ENUM_HASH = 0
+QF_ENUM_KEY = 1
 
 
 class GqfInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "nslots", "xnslots", "nblocks", "key_bits", "key_remainder_bits", "bits_per_slot",
         "range", "num_locks", "nregions", "total_size_in_bytes", "device_bytes", "region_blocks",
-        "sort_digit_bits")]
+        "sort_digit_bits", "value_bits")]
 
 
 _lib = None
@@ -116,6 +121,24 @@ def load_library(path=LIB_PATH):
     L.gqf_autotune.restype = ctypes.c_int
     L.gqf_group_autotune.argtypes = [_vp, ctypes.POINTER(_u64)]
     L.gqf_group_autotune.restype = ctypes.c_int
+    L.qf_malloc_device_kv.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
+    L.qf_enumerate.argtypes = [_vp, _u64, _u64, ctypes.c_int, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp]
+    L.qf_enumerate.restype = ctypes.c_int
+    L.qf_get_stats.argtypes = [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64), ctypes.POINTER(_u64)]
+    L.qf_get_stats.restype = ctypes.c_int
+    L.qf_cluster_census.argtypes = [_vp, _vp, _u64, _vp, ctypes.c_uint32, ctypes.POINTER(_u64),
+                                    ctypes.POINTER(_u64)]
+    L.qf_cluster_census.restype = ctypes.c_int
+    L.qf_serialize.argtypes = [_vp, ctypes.c_char_p]
+    L.qf_serialize.restype = _u64
+    L.qf_deserialize_device.argtypes = [ctypes.POINTER(_vp), ctypes.c_char_p]
+    L.qf_deserialize_device.restype = _u64
+    L.qf_resize_device.argtypes = [_vp, ctypes.c_int]
+    L.qf_resize_device.restype = ctypes.c_int64
+    L.qf_merge_device.argtypes = [_vp, _vp, _vp]
+    L.qf_merge_device.restype = ctypes.c_int64
+    L.qf_multi_merge_device.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _vp]
+    L.qf_multi_merge_device.restype = ctypes.c_int64
     _lib = L
     return L
 
@@ -142,19 +165,33 @@ def to_numpy_u64(t):
 
 
 class GQF:
-    """One device-resident counting quotient filter (2^nbits slots, r = 8)."""
+    """One device-resident counting quotient filter (2^nbits slots of 8
+    bits; value_bits > 0 gives the layout of a grown filter, see resize)."""
 
-    def __init__(self, nbits, bulk_config=True):
+    def __init__(self, nbits, bulk_config=True, value_bits=0, _handle=None):
         self._lib = load_library()
-        h = ctypes.c_void_p()
-        self._lib.qf_malloc_device(ctypes.byref(h), nbits, bulk_config)
-        if not h.value:
-            raise RuntimeError(f"qf_malloc_device failed for nbits={nbits}")
+        if _handle is None:
+            h = ctypes.c_void_p()
+            if value_bits:
+                self._lib.qf_malloc_device_kv(ctypes.byref(h), nbits, value_bits)
+            else:
+                self._lib.qf_malloc_device(ctypes.byref(h), nbits, bulk_config)
+            if not h.value:
+                raise RuntimeError(f"qf_malloc_device failed for nbits={nbits} value_bits={value_bits}")
+        else:
+            h = _handle
         self.handle = h
-        self.nbits = nbits
-        info = GqfInfo()
-        self._lib.qf_get_info(self.handle, ctypes.byref(info))
-        self.info = {n: int(getattr(info, n)) for n, _ in GqfInfo._fields_}
+        self.refresh_info()
+        self.nbits = self.info["nslots"].bit_length() - 1
+
+    @classmethod
+    def load(cls, path):
+        """A new filter from a file written by save() (qf_deserialize_device)."""
+        h = ctypes.c_void_p()
+        n = load_library().qf_deserialize_device(ctypes.byref(h), os.fsencode(path))
+        if not n or not h.value:
+            raise RuntimeError(f"qf_deserialize_device failed for {path}")
+        return cls(0, _handle=h)
 
     # -- lifecycle
     def close(self):
@@ -240,6 +277,77 @@ class GQF:
         info = GqfInfo()
         self._lib.qf_get_info(self.handle, ctypes.byref(info))
         self.info = {n: int(getattr(info, n)) for n, _ in GqfInfo._fields_}
+
+    # -- whole-table operations
+    def enumerate(self, q_lo=0, q_hi=None, form=QF_ENUM_HASH, values=False, stream=None):
+        """(hashes or keys, counts[, values]) int64 device tensors of the
+        items of quotients [q_lo, q_hi), in ascending hash order."""
+        import torch
+        if q_hi is None:
+            q_hi = self.info["nslots"]
+        n = _u64()
+        rc = self._lib.qf_enumerate(self.handle, q_lo, q_hi, form, None, None, None, 0, ctypes.byref(n),
+                                    _stream_ptr(stream))
+        if rc != 0:
+            raise RuntimeError(f"qf_enumerate rc={rc}")
+        m = int(n.value)
+        h = torch.empty(m, dtype=torch.int64, device="cuda")
+        c = torch.empty(m, dtype=torch.int64, device="cuda")
+        v = torch.empty(m if values else 0, dtype=torch.int64, device="cuda")
+        if m:
+            rc = self._lib.qf_enumerate(self.handle, q_lo, q_hi, form, _ptr(h), _ptr(c),
+                                        _ptr(v) if values else None, m, ctypes.byref(n), _stream_ptr(stream))
+            if rc != 0 or int(n.value) != m:
+                raise RuntimeError(f"qf_enumerate rc={rc}")
+        return (h, c, v) if values else (h, c)
+
+    def stats(self):
+        """{nelts, ndistinct_elts, noccupied_slots} computed from the table."""
+        a, b, c = _u64(), _u64(), _u64()
+        rc = self._lib.qf_get_stats(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        if rc != 0:
+            raise RuntimeError(f"qf_get_stats rc={rc}")
+        return {"nelts": int(a.value), "ndistinct_elts": int(b.value), "noccupied_slots": int(c.value)}
+
+    def cluster_census(self, nbins=0, lengths=True):
+        """(lengths int64 device tensor or None, histogram or None, max length)."""
+        import torch
+        nc, mx = _u64(), _u64()
+        rc = self._lib.qf_cluster_census(self.handle, None, 0, None, 0, ctypes.byref(nc), ctypes.byref(mx))
+        if rc != 0:
+            raise RuntimeError(f"qf_cluster_census rc={rc}")
+        lens = torch.empty(int(nc.value), dtype=torch.int64, device="cuda") if lengths else None
+        hist = torch.zeros(nbins, dtype=torch.int64, device="cuda") if nbins else None
+        if lengths or nbins:
+            rc = self._lib.qf_cluster_census(self.handle, _ptr(lens) if lengths else None,
+                                             lens.numel() if lengths else 0, _ptr(hist) if nbins else None,
+                                             nbins, ctypes.byref(nc), ctypes.byref(mx))
+            if rc != 0:
+                raise RuntimeError(f"qf_cluster_census rc={rc}")
+        return lens, hist, int(mx.value)
+
+    def save(self, path):
+        n = self._lib.qf_serialize(self.handle, os.fsencode(path))
+        if not n:
+            raise RuntimeError(f"qf_serialize failed for {path}")
+        return int(n)
+
+    def resize(self, new_nbits):
+        """Grow to 2^new_nbits slots keeping the hash; returns items moved."""
+        m = int(self._lib.qf_resize_device(self.handle, new_nbits))
+        if m < 0:
+            raise RuntimeError(f"qf_resize_device({new_nbits}) rc={m}")
+        self.refresh_info()
+        self.nbits = new_nbits
+        return m
+
+    def merge_from(self, *sources):
+        """self += every item of the sources (qf_multi_merge_device)."""
+        arr = (_vp * len(sources))(*[f.handle for f in sources])
+        m = int(self._lib.qf_multi_merge_device(arr, len(sources), self.handle))
+        if m < 0:
+            raise RuntimeError(f"qf_multi_merge_device rc={m}")
+        return m
 
     # -- status and image
     def status(self):

@@ -101,6 +101,7 @@ typedef struct gqf_info {
   uint64_t device_bytes;         /* bytes of the device table: 128 per block + 4096 slack */
   uint64_t region_blocks;        /* blocks per rebuild region (tuning) */
   uint64_t sort_digit_bits;      /* radix digit of the bulk sort, 0 = automatic (tuning) */
+  uint64_t value_bits;           /* 0, or the slot bits below the hash a resize added */
 } gqf_info;
 
 int qf_get_info(QF *qf, gqf_info *out);
@@ -154,6 +155,75 @@ int gqf_autotune(int nbits, double load, uint32_t *region_blocks, uint32_t *sort
 /* gqf_autotune for every filter of the group at the load its segment
  * brings, applied with qf_set_tuning */
 int gqf_group_autotune(gqf_group *g, const uint64_t *offsets);
+
+/* ---- whole-table operations ------------------------------------- */
+
+/* qf_malloc (src/gqf.cu:2073-2105) with value bits: 2^nbits slots of 8
+ * bits, key_bits = nbits + 8 - value_bits, value_bits 0..6 (the reference
+ * asserts key_remainder_bits >= 2, src/gqf.cu:1957).  Keys are stored as
+ * the slot hash (hash_64(key) << value_bits); values are always 0.  This
+ * is the layout qf_resize_device produces. */
+void qf_malloc_device_kv(QF **qf, int nbits, int value_bits);
+
+/* Output forms of qf_enumerate */
+#define QF_ENUM_HASH 0 /* the hash, (quotient << key_remainder_bits) | remainder (qfi_get_hash) */
+#define QF_ENUM_KEY 1  /* the key, hash_64i of the hash (qfi_get_key, src/gqf.cu:3791-3805) */
+
+/* Items of the runs of quotients in [q_lo, q_hi), in ascending hash order:
+ * the sequence the reference's iterator visits from
+ * qf_iterator_from_position(q_lo) (src/gqf.cu:3655-3700, qfi_next
+ * :3814-3840), one (hash or key, count, value) per counter.  hashes, counts
+ * and values are device arrays of cap entries, each may be NULL; *n_items
+ * (host) receives the number of items in the range, of which the first
+ * min(n_items, cap) are written.  q_hi is clamped to nslots.  Synchronises
+ * `stream`.  Returns 0, or < 0 for a bad handle or form. */
+int qf_enumerate(QF *qf, uint64_t q_lo, uint64_t q_hi, int form, uint64_t *hashes, uint64_t *counts,
+                 uint64_t *values, uint64_t cap, uint64_t *n_items, void *stream);
+
+/* The metadata counters the reference declares but whose GPU updates are
+ * commented out (src/gqf.cu:1094, 1384-1401, 1545-1547; read by
+ * qf_get_sum_of_counts, qf_get_num_distinct_key_value_pairs and
+ * qf_get_num_occupied_slots, src/gqf.cu:3619-3643), computed from the
+ * table: sum of counts, items, and slots the runs use.  Synchronises. */
+int qf_get_stats(QF *qf, uint64_t *nelts, uint64_t *ndistinct_elts, uint64_t *noccupied_slots);
+
+/* Cluster census (src/cluster_length_test.cu:37-120): the lengths of the
+ * maximal stretches of used slots, in slot order.  lengths: device array
+ * of cap entries or NULL; hist: device array of nbins (<= 4096) u64
+ * counters or NULL, ADDED to (bin i counts clusters of length i, the last
+ * bin those of length >= nbins - 1); *num_clusters and *max_len are host
+ * outputs, either may be NULL.  Allocates 8 bytes per 64-slot block for
+ * the call.  Synchronises. */
+int qf_cluster_census(QF *qf, uint64_t *lengths, uint64_t cap, uint64_t *hist, uint32_t nbins,
+                      uint64_t *num_clusters, uint64_t *max_len);
+
+/* File format: the reference's qfmetadata (120 bytes,
+ * include/gqf_int.cuh:96-113, magic 1018874902021329732) followed by the
+ * packed 81-byte blocks -- the buffer qf_use maps (src/gqf.cu:2025-2053),
+ * for the qf_serialize / qf_deserialize the reference declares
+ * (include/gqf_file.cuh:45-49).  nelts, ndistinct_elts and noccupied_slots
+ * are written from qf_get_stats.  Returns the bytes written, 0 on error. */
+uint64_t qf_serialize(QF *qf, const char *filename);
+/* A new device filter from such a file; the metadata (magic, invertible
+ * hash mode, 8-bit slots, sizes) and the block image are validated.
+ * Returns the bytes read, 0 on error (*qf = NULL then). */
+uint64_t qf_deserialize_device(QF **qf, const char *filename);
+
+/* Grows the table to 2^new_nbits slots keeping the hash (key_bits): every
+ * (hash, count) moves to the larger table, as qf_resize_malloc re-inserts
+ * them (src/gqf.cu:2301-2330).  The freed remainder bits become value
+ * bits, so new_nbits - nbits + value_bits must stay <= 6.  The handle stays
+ * valid; the table and workspace are reallocated.  Returns the number of
+ * items moved, or < 0 on error (the filter is unchanged then). */
+int64_t qf_resize_device(QF *qf, int new_nbits);
+
+/* qfc += every item of qfa and qfb, counts added (qf_merge,
+ * src/gqf.cu:3896-3938).  All filters must have the same key_bits, and qfc
+ * at least the value bits of each input.  Returns the number of items
+ * inserted, or < 0 on error. */
+int64_t qf_merge_device(QF *qfa, QF *qfb, QF *qfc);
+/* qfr += every item of the nqf filters (qf_multi_merge, src/gqf.cu:3943-4008) */
+int64_t qf_multi_merge_device(QF *const *qfs, int nqf, QF *qfr);
 
 /* last HIP error seen by the library (0 = none); clears it */
 int qf_last_error(void);

@@ -64,16 +64,17 @@ struct quotient_filter {
   uint64_t nslots;              // 2^q
   uint64_t xnslots;             // nslots + 10 sqrt(nslots), truncated (src/gqf.cu:1950)
   uint64_t nblocks;             // ceil(xnslots / 64)
-  uint64_t key_bits;            // q + 8
-  uint64_t key_remainder_bits;  // 8
+  uint64_t key_bits;            // hash bits: q + 8 - value_bits
+  uint64_t key_remainder_bits;  // 8 - value_bits
   uint64_t bits_per_slot;       // 8
-  uint64_t range;               // 2^(q+8)
+  uint64_t range;               // slot-hash range 2^(q+8) (the reference's range is 2^key_bits)
   uint64_t num_locks;           // xnslots / 8192 + 2 (src/gqf.cu:1996)
   uint64_t nregions;            // ceil(nblocks / region_blocks): even/odd rebuild regions
   uint64_t region_blocks;       // blocks per rebuild region (tunable, 16..56; default 48)
   uint8_t* blocks;              // nblocks * GQF_BLOCK_BYTES bytes (+ slack)
   uint32_t* locks;              // point-API region locks: num_locks words, 64 apart
   uint32_t* status;             // GQF_STATUS_* bits, sticky until cleared
+  uint64_t value_bits;          // 0, or the low slot bits a resize freed (see gqf_key_hash)
 };
 
 
@@ -109,6 +110,55 @@ __host__ __device__ static inline uint64_t gqf_hash64(uint64_t key, uint64_t mas
   key = key ^ (key >> 28);
   key = (key + (key << 31)) & mask;
   return key;
+}
+
+// Inverse of an odd multiplier modulo 2^64 by Newton's iteration
+// x <- x (2 - c x): each step doubles the number of correct low bits, and
+// x = c is already right in the low 3 bits (c c = 1 mod 8 for odd c).
+__host__ __device__ constexpr uint64_t gqf_inv_odd(uint64_t c) {
+  uint64_t x = c;
+  for (int i = 0; i < 5; i++) x *= 2 - c * x;
+  return x;
+}
+
+// Undoes y = x ^ (x >> s) on 64-bit words: the top s bits of x equal those
+// of y, and each further pass x = y ^ (x >> s) fixes s more bits.
+__host__ __device__ static inline uint64_t gqf_unxorshift(uint64_t y, int s) {
+  uint64_t x = y;
+  for (int i = s; i < 64; i += s) x = y ^ (x >> s);
+  return x;
+}
+
+// Inverse of gqf_hash64 under the same mask, for hashes < mask + 1
+// (the key the reference's qfi_get_key recovers with hash_64i,
+// src/hashutil.cu:144-181, src/gqf.cu:3791-3805).  The mix is a chain of
+// multiplications by odd constants and xors with right shifts of a masked
+// value; multiplication commutes with masking modulo a power of two, so
+// each step is undone in reverse order: a multiplication through the
+// constant's inverse modulo 2^64, an xor-shift by gqf_unxorshift.
+//   forward: k (2^21 - 1) - 1,  ^ >>24,  * 265,  ^ >>14,  * 21,  ^ >>28,  * (2^31 + 1)
+__host__ __device__ static inline uint64_t gqf_hash64_inv(uint64_t h, uint64_t mask) {
+  constexpr uint64_t i31 = gqf_inv_odd((1ULL << 31) + 1);
+  constexpr uint64_t i21 = gqf_inv_odd(21);
+  constexpr uint64_t i265 = gqf_inv_odd(265);
+  constexpr uint64_t ik = gqf_inv_odd((1ULL << 21) - 1);
+  uint64_t k = (h * i31) & mask;
+  k = gqf_unxorshift(k, 28);
+  k = (k * i21) & mask;
+  k = gqf_unxorshift(k, 14);
+  k = (k * i265) & mask;
+  k = gqf_unxorshift(k, 24);
+  return ((k + 1) * ik) & mask;
+}
+
+// Slot hash of a key: the reference's (hash_64(key) << value_bits) | value
+// with value 0 (qf_insert / qf_count_key_value, src/gqf.cu:2475-2480,
+// 3484-3492); quotient = slot hash >> 8, remainder = its low 8 bits.
+// value_bits is 0 for every filter qf_malloc_device makes; a filter grown by
+// qf_resize_device keeps its hash width (key_bits) and gains value bits, so
+// its remainders carry that many zero bits below the hash's.
+__host__ __device__ static inline uint64_t gqf_key_hash(const quotient_filter& f, uint64_t key) {
+  return gqf_hash64(key, gqf_mask(f.key_bits)) << f.value_bits;
 }
 
 // ---------------------------------------------------------------------------
@@ -538,9 +588,11 @@ __host__ __device__ static inline qf_returns insert1_if_not_exists(const QF& f, 
   return QF_ITEM_INSERTED;
 }
 
+// The reference reduces the (hashed or caller-given) key modulo its range
+// 2^key_bits (src/gqf.cu:2973-2980) and ignores the point value.
 __host__ __device__ static inline uint64_t key_to_hash(const QF& f, uint64_t key, uint8_t flags) {
   if (!(flags & QF_KEY_IS_HASH)) key = gqf_hash64(key, gqf_mask(f.key_bits));
-  return key % f.range;
+  return (key & gqf_mask(f.key_bits)) << f.value_bits;
 }
 
 __device__ static inline bool try_lock(const QF& f, uint64_t idx) {

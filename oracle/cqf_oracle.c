@@ -1089,6 +1089,26 @@ uint64_t orc_enumerate(const orc_qf *qf, uint64_t *hashes, uint64_t *counts, uin
     return m;
 }
 
+/* Cluster census: one_thread_cluster_write (src/cluster_length_test.cu:82-120)
+ * over first_empty_slot_wrapper -> find_first_empty_slot (src/gqf.cu:728-776):
+ * from each slot, the first empty slot at or after it; a used slot starts a
+ * cluster that ends there. */
+uint64_t orc_cluster_census(orc_qf *qf, uint64_t *lengths, uint64_t cap)
+{
+    uint64_t i = 0, start = 0;
+    while (start < qf->xnslots) {
+        uint64_t next = find_first_empty_slot(qf, start);
+        if (start == next) {
+            start++;
+        } else {
+            if (i < cap) lengths[i] = next - start;
+            start = next;
+            i++;
+        }
+    }
+    return i;
+}
+
 /* ------------------------------------------------------------------ */
 /* generators */
 

@@ -26,6 +26,10 @@
  *                        gives the zero image
  *   counts               brute-force masked multisets
  *   sizing               closed-form table
+ *   enumerate            the multiset the image was built from (canonical
+ *                        images of spec_model.py and of counted inserts)
+ *   cluster census       maximal used-slot stretches of layouts built by
+ *                        spec_model.py from the format's invariants
  *   insert1_if_not_exists  return codes and returnedVal on fixed cases;
  *                        PARITY UNPINNED for its slot-wise search on runs
  *                        holding multi-slot counters (restated literally from
@@ -120,6 +124,10 @@ int      orc_build_canonical(orc_qf *qf, const uint64_t *hashes, const uint64_t 
 /* Enumerates (hash, count) items of the image in ascending hash order using
  * the reference decoder; returns the number written (<= cap). */
 uint64_t orc_enumerate(const orc_qf *qf, uint64_t *hashes, uint64_t *counts, uint64_t cap);
+
+/* Cluster lengths in slot order (the reference's census loop); returns the
+ * number of clusters, writes the first cap lengths. */
+uint64_t orc_cluster_census(orc_qf *qf, uint64_t *lengths, uint64_t cap);
 
 /* ---- key generators ---- */
 void     orc_splitmix64_fill(uint64_t seed, uint64_t skip, uint64_t n, uint64_t *out);

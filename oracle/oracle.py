@@ -85,6 +85,8 @@ def lib():
         L.orc_build_canonical.restype = ctypes.c_int
         L.orc_enumerate.argtypes = [P, _u64p, _u64p, ctypes.c_uint64]
         L.orc_enumerate.restype = ctypes.c_uint64
+        L.orc_cluster_census.argtypes = [P, _u64p, ctypes.c_uint64]
+        L.orc_cluster_census.restype = ctypes.c_uint64
         L.orc_splitmix64_fill.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u64p]
         L.orc_time_insert_query.argtypes = [ctypes.c_int, _u64p, ctypes.c_uint64, _u64p,
                                             ctypes.c_uint64, _u64p,
@@ -253,6 +255,14 @@ class OracleQF:
         c = np.empty(cap, dtype=np.uint64)
         m = int(lib().orc_enumerate(ctypes.byref(self._s), _p(h), _p(c), cap))
         return h[:m], c[:m]
+
+
+    def cluster_census(self):
+        """Cluster lengths in slot order (the reference's census loop)."""
+        cap = self.xnslots
+        out = np.empty(cap, dtype=np.uint64)
+        m = int(lib().orc_cluster_census(ctypes.byref(self._s), _p(out), cap))
+        return out[:m]
 
 
 def canonical_image(nbits, hashes, counts):

@@ -103,6 +103,39 @@ def build_image(nbits, multiset):
     return bytes(img)
 
 
+def used_slots(nbits, multiset):
+    """0/1 per slot of the table (nblocks * 64): 1 where a run of the layout
+    the multiset fixes lies (runs placed as in build_image)."""
+    nslots, xnslots, nblocks = sizes(nbits)
+    used = [0] * (nblocks * 64)
+    by_q = {}
+    for h, c in multiset.items():
+        if c > 0:
+            by_q.setdefault(h >> 8, []).append((h & 0xFF, c))
+    prev_end = -1
+    for q in sorted(by_q):
+        n = sum(len(encode(rem, c)) for rem, c in by_q[q])
+        start = max(q, prev_end + 1)
+        for s in range(start, start + n):
+            used[s] = 1
+        prev_end = start + n - 1
+    return used
+
+
+def clusters(used):
+    """Lengths of the maximal stretches of 1s, left to right."""
+    out, run = [], 0
+    for u in used:
+        if u:
+            run += 1
+        elif run:
+            out.append(run)
+            run = 0
+    if run:
+        out.append(run)
+    return out
+
+
 def hash_64(key, mask):
     """Thomas Wang's 64-bit mix written with multiplications
     ((key << 21) - key - 1, *265, *21, *(2^31 + 1)), not the reference's

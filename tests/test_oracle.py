@@ -14,6 +14,8 @@ is checked against something independent of its C code:
     (tests/spec_model.py) compared byte for byte with the oracle's sequential
     bulk insert / reduce / delete;
   - counts: brute-force multisets of masked keys;
+  - enumeration and cluster census: the multiset and the used-slot stretches
+    of layouts built by the spec model;
   - sizing: the closed-form table of xnslots / nblocks / bytes / locks.
 """
 import numpy as np
@@ -187,6 +189,60 @@ def test_canonical_builder_matches_spec(oracle):
     cs = rng.choice([1, 2, 3, 4, 9, 300, 65000], size=hs.size).astype(np.uint64)
     img = oracle.canonical_image(nbits, hs, cs)
     assert bytes(img) == S.build_image(nbits, {int(h): int(c) for h, c in zip(hs, cs)})
+
+
+# ---------------------------------------------------------------------------
+# enumeration and cluster census against layouts built from the invariants
+
+def _spec_multiset(rng, nbits, n, counts=(1,)):
+    hs = np.unique(rng.integers(0, 1 << (nbits + 8), size=n).astype(np.uint64))
+    cs = rng.choice(counts, size=hs.size)
+    return {int(h): int(c) for h, c in zip(hs, cs)}
+
+
+@pytest.mark.parametrize("nbits,frac,counts", [(8, 0.5, (1,)), (10, 0.9, (1,)), (10, 0.25, (1, 2, 3, 7, 1000)),
+                                               (12, 0.95, (1,)), (11, 0.3, (1, 70000))])
+def test_enumerate_returns_the_multiset(oracle, nbits, frac, counts):
+    rng = np.random.default_rng(nbits * 100 + len(counts))
+    ms = _spec_multiset(rng, nbits, int(frac * (1 << nbits)), counts)
+    try:
+        img = S.build_image(nbits, ms)
+    except ValueError:
+        pytest.skip("layout does not fit")
+    o = oracle.OracleQF(nbits)
+    o.set_image(np.frombuffer(img, dtype=np.uint8))
+    h, c = o.enumerate()
+    assert h.tolist() == sorted(ms)
+    assert c.tolist() == [ms[k] for k in sorted(ms)]
+    o.close()
+
+
+@pytest.mark.parametrize("nbits,frac,counts", [(8, 0.9, (1,)), (10, 0.5, (1,)), (10, 0.95, (1,)),
+                                               (12, 0.3, (1, 2, 3, 400)), (9, 0.2, (1, 70000))])
+def test_cluster_census_matches_layout(oracle, nbits, frac, counts):
+    rng = np.random.default_rng(nbits * 7 + len(counts))
+    ms = _spec_multiset(rng, nbits, int(frac * (1 << nbits)), counts)
+    try:
+        img = S.build_image(nbits, ms)
+    except ValueError:
+        pytest.skip("layout does not fit")
+    o = oracle.OracleQF(nbits)
+    o.set_image(np.frombuffer(img, dtype=np.uint8))
+    lens = o.cluster_census().tolist()
+    assert lens == S.clusters(S.used_slots(nbits, ms)[:S.sizes(nbits)[1]])
+    assert sum(lens) == sum(S.used_slots(nbits, ms))
+    o.close()
+
+
+def test_cluster_census_special_cases(oracle):
+    o = oracle.OracleQF(10)
+    assert o.cluster_census().tolist() == []
+    # two runs that touch form one cluster; a gap splits them
+    img = S.build_image(10, {(5 << 8) | 1: 1, (5 << 8) | 2: 1, (6 << 8) | 9: 3, (20 << 8): 1})
+    o.set_image(np.frombuffer(img, dtype=np.uint8))
+    # quotient 5: slots 5,6; quotient 6: slots 7..9 (0 9 9 -> r 0 r); quotient 20: slot 20
+    assert o.cluster_census().tolist() == [5, 1]
+    o.close()
 
 
 # ---------------------------------------------------------------------------
