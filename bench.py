@@ -44,11 +44,13 @@ def parse():
     ap.add_argument("--cpu-nbits", type=int, default=26,
                     help="table size of the CPU baseline sample (2^26: about 25 s on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_q28.md"),
-                    help="rocprofv3 counter summary holding the measured HBM bytes per launch")
-    ap.add_argument("--workload", choices=["bulk", "zipf", "groups", "serve"], default="bulk",
+    ap.add_argument("--traffic", default=None,
+                    help="rocprofv3 counter summary holding the measured HBM bytes per launch "
+                         "(default: profiles/traffic_q28.md, profiles/traffic_table_q28.md for --workload table)")
+    ap.add_argument("--workload", choices=["bulk", "zipf", "groups", "serve", "table"], default="bulk",
                     help="bulk: uniform insert/get/delete (default); zipf: skewed bulk_insert_reduce; "
-                         "groups: 256 k-mer filters in one persistent launch; serve: served request stream")
+                         "groups: 256 k-mer filters in one persistent launch; serve: served request stream; "
+                         "table: enumeration, census, statistics, file image and growth of a full table")
     ap.add_argument("--serve-requests", type=int, default=6000)
     ap.add_argument("--serve-rate", type=float, default=2.0e5, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
@@ -354,8 +356,191 @@ def serve_main(args):
     print(json.dumps(out), flush=True)
 
 
+def table_cpu_baseline(nbits, seed):
+    """The oracle's restatement of the reference iterator (orc_enumerate:
+    qf_iterator_from_position + qfi_next + qfi_get_hash) and census loop
+    (orc_cluster_census) over a 2^nbits-slot table at 90% load of
+    splitmix64(seed) keys, built from its multiset; one core."""
+    from oracle import oracle as O
+    import numpy as np
+    n = int(0.9 * (1 << nbits))
+    hs = O.hash_many(O.splitmix64(seed, n), (1 << (nbits + 8)) - 1)
+    u, c = np.unique(hs, return_counts=True)
+    o = O.OracleQF(nbits)
+    o.build_canonical(u, c.astype(np.uint64))
+    t0 = time.perf_counter()
+    h, cc = o.enumerate()
+    t_enum = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lens = o.cluster_census()
+    t_census = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": round(h.size / t_enum / 1e6, 3),
+        "unit": "M items/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"a 2^{nbits}-slot table at 90% load ({n} splitmix64(seed={seed}) keys, {h.size} items): "
+                   f"the reference iterator restated in C walks every item once; single thread"),
+        "census_mslots_per_s": round(f_xn(nbits) / t_census / 1e6, 3),
+        "clusters": int(lens.size),
+        "seconds": round(t_enum + t_census, 3),
+        "cpu_model": platform.processor() or platform.machine(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def table_main(args):
+    """Whole-table operations on a 2^28-slot table at 90% load (the bulk
+    workload's inserted keys, splitmix64 seed 3): enumeration of every
+    (hash, count) item in hash order -- the reference iterator's sequence,
+    as one device pass --, the table statistics, the cluster census, the
+    file image written and read back, and growth to 2^29 slots."""
+    import ctypes
+    import tempfile
+    import numpy as np
+    import torch
+    from cqf_amd import gqf
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    nbits = args.nbits
+    n_ins = int(args.load * (1 << nbits))
+    batch = 1 << args.batch_log2
+    f = gqf.GQF(nbits)
+    f.reserve(batch)
+    keys = torch.empty(n_ins, dtype=torch.int64, device=dev)
+    gqf.fill_splitmix64(keys, args.seed, 0, stream)
+    work = keys.clone()
+    for b in range(0, n_ins, batch):
+        f.bulk_insert(work[b:b + batch], 0, stream)
+    del work
+    torch.cuda.synchronize()
+    st = f.stats()
+    m = st["ndistinct_elts"]
+    L = gqf.load_library()
+    sp = gqf._stream_ptr(stream)
+    hashes = torch.empty(m, dtype=torch.int64, device=dev)
+    counts = torch.empty(m, dtype=torch.int64, device=dev)
+    nn, nc, mx = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    L.qf_cluster_census(f.handle, None, 0, None, 0, ctypes.byref(nc), ctypes.byref(mx))
+    n_clusters = int(nc.value)
+    lens = torch.empty(n_clusters, dtype=torch.int64, device=dev)
+    hist = torch.zeros(256, dtype=torch.int64, device=dev)
+
+    def enum_once():
+        rc = L.qf_enumerate(f.handle, 0, 1 << nbits, gqf.QF_ENUM_HASH, gqf._ptr(hashes), gqf._ptr(counts), None,
+                            m, ctypes.byref(nn), sp)
+        assert rc == 0 and nn.value == m
+
+    def census_once():
+        hist.zero_()
+        rc = L.qf_cluster_census(f.handle, gqf._ptr(lens), n_clusters, gqf._ptr(hist), 256, ctypes.byref(nc),
+                                 ctypes.byref(mx))
+        assert rc == 0 and nc.value == n_clusters
+
+    for _ in range(args.warmup):
+        enum_once()
+        census_once()
+        f.stats()
+    torch.cuda.synchronize()
+    gqf.profile_enable(True)
+    t_enum = t_census = t_stats = 0.0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        enum_once()
+        t_enum += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        census_once()
+        t_census += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        f.stats()
+        t_stats += time.perf_counter() - t0
+    kern = {k: gqf.profile_read(k) for k in ("enum_count", "enum_emit", "census_mask", "census_write")}
+    gqf.profile_enable(False)
+
+    # checks on the measured outputs
+    sorted_ok = bool((hashes[1:] > hashes[:-1]).all().item()) if m > 1 else True
+    nelts_ok = int(counts.sum().item()) == n_ins == st["nelts"]
+    census_ok = int(lens.sum().item()) == st["noccupied_slots"] and int(hist.sum().item()) == n_clusters
+
+    # file image out and back (host and disk bound)
+    path = os.path.join(tempfile.gettempdir(), f"gqf_bench_{os.getpid()}.cqf")
+    t0 = time.perf_counter()
+    nbytes = f.save(path)
+    t_save = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = gqf.GQF.load(path)
+    t_load = time.perf_counter() - t0
+    os.remove(path)
+    sample = keys[:1 << 20].contiguous()
+    before = f.bulk_get(sample)
+    load_ok = bool(torch.equal(g.bulk_get(sample), before))
+    g.close()
+
+    # growth to 2^(q+1) slots: every item moved to the larger table
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    moved = f.resize(nbits + 1)
+    t_resize = time.perf_counter() - t0
+    resize_ok = moved == m and bool(torch.equal(f.bulk_get(sample), before)) and f.status() == 0
+
+    nb_q = (1 << nbits) // 64
+    unit_bytes = {"enum_count": 81 * nb_q, "enum_emit": 81 * nb_q + 16 * m,
+                  "census_mask": 17 * f_nblocks(nbits), "census_write": 8 * n_clusters}
+    cands = {k: v for k, v in kern.items() if v[1]}
+    dom = max(cands, key=lambda k: cands[k][0])
+    d_ms, d_cnt = cands[dom]
+    achieved = unit_bytes[dom] / (d_ms / d_cnt / 1e3) / 1e9
+    traffic = read_traffic(args.traffic, dom)
+    value = m * args.steps / t_enum / 1e6
+    out = {
+        "metric": "M items/s enumerated in hash order (device iterator), 2^28 slots, 90% load",
+        "value": round(value, 3), "unit": "M items/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((t_enum + t_census + t_stats) / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": (f"whole-table passes over 2^{nbits} slots holding {n_ins} splitmix64(seed="
+                                f"{args.seed}) keys ({m} items): qf_enumerate of every item (hash, count) into "
+                                f"device arrays, qf_cluster_census (lengths + 256-bin histogram), qf_get_stats; "
+                                f"once: qf_serialize / qf_deserialize_device, qf_resize_device to 2^{nbits + 1}"),
+                   "nbits": nbits, "items": m, "clusters": n_clusters, "seed": args.seed},
+        "enumerate_ms": round(t_enum / args.steps * 1e3, 3),
+        "census_ms": round(t_census / args.steps * 1e3, 3),
+        "census_mslots_per_s": round(f_xn(nbits) * args.steps / t_census / 1e6, 1),
+        "stats_ms": round(t_stats / args.steps * 1e3, 3),
+        "save_s": round(t_save, 3), "load_s": round(t_load, 3), "file_bytes": nbytes,
+        "resize_s": round(t_resize, 3), "resize_items_per_s_m": round(moved / t_resize / 1e6, 1),
+        "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launches": d_cnt, "avg_launch_ms": round(d_ms / d_cnt, 4),
+                     "bytes_per_unit": {"enum_count": "81 per block", "enum_emit": "81 per block + 16 per item",
+                                        "census_mask": "17 per block", "census_write": "8 per cluster"}[dom]},
+        "checks": {"hash_order": sorted_ok, "sum_of_counts_equals_inserts": nelts_ok, "census_sums": census_ok,
+                   "max_cluster": int(mx.value), "file_round_trip": load_ok, "resize_counts_kept": resize_ok,
+                   "status": f.status()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = table_cpu_baseline(nbits, args.seed)
+    f.close()
+    print(json.dumps(out), flush=True)
+
+
+def f_xn(nbits):
+    n = 1 << nbits
+    return n + int(10 * (n ** 0.5))
+
+
+def f_nblocks(nbits):
+    return (f_xn(nbits) + 63) // 64
+
+
 def main():
     args = parse()
+    if args.traffic is None:
+        name = "traffic_table_q28.md" if args.workload == "table" else "traffic_q28.md"
+        args.traffic = os.path.join(ROOT, "profiles", name)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(filters shard by hash prefix or by filter; see the design notes)", file=sys.stderr)
@@ -372,6 +557,9 @@ def main():
         return
     if args.workload == "serve":
         serve_main(args)
+        return
+    if args.workload == "table":
+        table_main(args)
         return
 
     import numpy as np

@@ -676,6 +676,14 @@ struct EnumScratch {
   uint32_t* wg_items = nullptr;
   uint64_t* wg_base = nullptr;           // nwg + 1
   unsigned long long* stats = nullptr;   // items, sum of counts, used slots, spare
+  // marks of item starts for the write pass: one word per block from the
+  // first enumerated block to the end of the table
+  uint64_t nwords = 0;
+  unsigned long long* starts = nullptr;
+  unsigned long long* max_word = nullptr;
+  uint32_t* local_prefix = nullptr;
+  uint32_t* group_sums = nullptr;
+  uint64_t* group_base = nullptr;
   bool init(uint64_t nblocks) {
     nwg = gqf_enum_wg_count(nblocks);
     dev_alloc(&wg_items, nwg);
@@ -683,10 +691,27 @@ struct EnumScratch {
     dev_alloc(&stats, 4);
     return wg_items && wg_base && stats;
   }
+  bool init_marks(uint64_t words, hipStream_t s) {
+    nwords = words;
+    const uint64_t ng = gqf_enum_wg_count(words);
+    dev_alloc(&starts, words);
+    dev_alloc(&max_word, 1);
+    dev_alloc(&local_prefix, words);
+    dev_alloc(&group_sums, ng);
+    dev_alloc(&group_base, ng + 1);
+    if (!starts || !max_word || !local_prefix || !group_sums || !group_base) return false;
+    GQF_CHECK(hipMemsetAsync(starts, 0, words * sizeof(unsigned long long), s));
+    return true;
+  }
   ~EnumScratch() {
     dev_free(wg_items);
     dev_free(wg_base);
     dev_free(stats);
+    dev_free(starts);
+    dev_free(max_word);
+    dev_free(local_prefix);
+    dev_free(group_sums);
+    dev_free(group_base);
   }
 };
 
@@ -703,7 +728,7 @@ int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF
   EnumScratch sc;
   if (!sc.init(nb)) return -3;
   GQF_CHECK(hipMemsetAsync(sc.stats, 0, 4 * sizeof(unsigned long long), s));
-  gqf_launch_enum_count(src_dev, 0, nb, 0, src.nslots, sc.wg_items, sc.wg_base, sc.stats, s);
+  gqf_launch_enum_count(src_dev, 0, nb, 0, src.nslots, sc.wg_items, sc.wg_base, sc.stats, nullptr, nullptr, s);
   std::vector<uint64_t> base(sc.nwg + 1);
   GQF_CHECK(hipMemcpyAsync(base.data(), sc.wg_base, base.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   GQF_CHECK(hipStreamSynchronize(s));
@@ -722,6 +747,11 @@ int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF
     return -3;
   }
   init_region_workspace(dst_ws, dst.nblocks);
+  if (!sc.init_marks(src.nblocks, s)) {
+    dev_free(items);
+    dev_free(counts);
+    return -3;
+  }
   for (uint64_t u = 0; u < sc.nwg;) {
     uint64_t v = u + 1;
     while (v < sc.nwg && base[v + 1] - base[u] <= cap) v++;
@@ -729,9 +759,17 @@ int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF
     if (n) {
       const uint64_t b_lo = u * GQF_ENUM_WG_BLOCKS, b_hi = v * GQF_ENUM_WG_BLOCKS < nb ? v * GQF_ENUM_WG_BLOCKS : nb;
       const uint64_t q_hi = b_hi * 64 < src.nslots ? b_hi * 64 : src.nslots;
-      gqf_launch_enum_write(src_dev, b_lo, b_hi, b_lo * 64, q_hi, sc.wg_base + u, base[u], GQF_ENUM_OUT_MOVE,
-                            (int)src.value_bits, (int)dst.value_bits, gqf_mask(src.key_bits), items, counts, nullptr,
-                            cap, s);
+      GQF_CHECK(hipMemsetAsync(sc.max_word, 0, sizeof(unsigned long long), s));
+      unsigned long long* marks = sc.starts + b_lo;  // word 0 of this pass is block b_lo
+      gqf_launch_enum_count(src_dev, b_lo, b_hi, b_lo * 64, q_hi, sc.wg_items, sc.wg_base, sc.stats, marks,
+                            sc.max_word, s);
+      // the write pass covers the marked words only: up to the last one
+      unsigned long long last = 0;
+      GQF_CHECK(hipMemcpyAsync(&last, sc.max_word, sizeof(last), hipMemcpyDeviceToHost, s));
+      GQF_CHECK(hipStreamSynchronize(s));
+      gqf_launch_enum_emit(src_dev, b_lo, (uint64_t)last + 1, marks, sc.max_word, sc.local_prefix, sc.group_sums,
+                           sc.group_base, GQF_ENUM_OUT_MOVE, (int)src.value_bits, (int)dst.value_bits,
+                           gqf_mask(src.key_bits), items, counts, nullptr, cap, s);
       gqf_launch_region_bounds(dst_dev, dst.nregions, items, n, nullptr, dst_ws.region_start, dst_ws.region_list,
                                dst_ws.list_count, s);
       gqf_launch_rebuild(dst_dev, dst.nregions, items, counts, n, dst_ws.region_start, dst_ws.region_list,
@@ -781,13 +819,21 @@ extern "C" int qf_enumerate(QF* qf, uint64_t q_lo, uint64_t q_hi, int form, uint
   if (q_lo >= q_hi) return 0;
   const uint64_t b_lo = q_lo / 64, b_hi = (q_hi + 63) / 64;
   EnumScratch sc;
-  if (!sc.init(b_hi - b_lo)) return -3;
+  const bool write = cap && (hashes || counts || values);
+  if (!sc.init(b_hi - b_lo) || (write && !sc.init_marks(h.nblocks - b_lo, s))) return -3;
   GQF_CHECK(hipMemsetAsync(sc.stats, 0, 4 * sizeof(unsigned long long), s));
-  gqf_launch_enum_count(f->dev, b_lo, b_hi, q_lo, q_hi, sc.wg_items, sc.wg_base, sc.stats, s);
-  if (cap && (hashes || counts || values))
-    gqf_launch_enum_write(f->dev, b_lo, b_hi, q_lo, q_hi, sc.wg_base, 0,
-                          form == QF_ENUM_KEY ? GQF_ENUM_OUT_KEY : GQF_ENUM_OUT_HASH, (int)h.value_bits, 0,
-                          gqf_mask(h.key_bits), hashes, counts, values, cap, s);
+  if (write) GQF_CHECK(hipMemsetAsync(sc.max_word, 0, sizeof(unsigned long long), s));
+  gqf_launch_enum_count(f->dev, b_lo, b_hi, q_lo, q_hi, sc.wg_items, sc.wg_base, sc.stats, sc.starts, sc.max_word,
+                        s);
+  unsigned long long last = 0;
+  if (write) {
+    GQF_CHECK(hipMemcpyAsync(&last, sc.max_word, sizeof(last), hipMemcpyDeviceToHost, s));
+    GQF_CHECK(hipStreamSynchronize(s));
+  }
+  if (write)
+    gqf_launch_enum_emit(f->dev, b_lo, (uint64_t)last + 1, sc.starts, sc.max_word, sc.local_prefix, sc.group_sums,
+                         sc.group_base, form == QF_ENUM_KEY ? GQF_ENUM_OUT_KEY : GQF_ENUM_OUT_HASH,
+                         (int)h.value_bits, 0, gqf_mask(h.key_bits), hashes, counts, values, cap, s);
   uint64_t total = 0;
   hipError_t e = hipMemcpyAsync(&total, sc.wg_base + sc.nwg, sizeof(total), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -805,7 +851,8 @@ extern "C" int qf_get_stats(QF* qf, uint64_t* nelts, uint64_t* ndistinct_elts, u
   if (!sc.init(nb)) return -3;
   GQF_CHECK(hipDeviceSynchronize());
   GQF_CHECK(hipMemset(sc.stats, 0, 4 * sizeof(unsigned long long)));
-  gqf_launch_enum_count(f->dev, 0, nb, 0, f->host.nslots, sc.wg_items, sc.wg_base, sc.stats, nullptr);
+  gqf_launch_enum_count(f->dev, 0, nb, 0, f->host.nslots, sc.wg_items, sc.wg_base, sc.stats, nullptr, nullptr,
+                        nullptr);
   unsigned long long st[4] = {0, 0, 0, 0};
   hipError_t e = hipMemcpy(st, sc.stats, sizeof(st), hipMemcpyDeviceToHost);
   note_error(e, "qf_get_stats");

@@ -55,8 +55,9 @@ __device__ inline void load_row(const QF& f, uint64_t b, uint32_t* row) {
 }
 
 // Visits the counters of the runs of block b's occupied quotients in
-// [q_lo, q_hi), in table order: item(storage hash, count).  Returns the
-// slots those runs use, or -1 if a run has no run end (a damaged table).
+// [q_lo, q_hi), in table order: item(storage hash, count, first slot).
+// Returns the slots those runs use, or -1 if a run has no run end (a
+// damaged table).
 template <typename Item>
 __device__ inline int64_t walk_items(const QF& f, uint64_t b, uint64_t q_lo, uint64_t q_hi, const slot_row& sl,
                                      Item item) {
@@ -78,7 +79,7 @@ __device__ inline int64_t walk_items(const QF& f, uint64_t b, uint64_t q_lo, uin
     while (idx < len) {
       uint64_t rem, cnt;
       const uint32_t last = gqf_decode_counter(at, len, idx, &rem, &cnt);
-      item((q << 8) | rem, cnt);
+      item((q << 8) | rem, cnt, start + idx);
       idx = last + 1;
     }
   }
@@ -123,19 +124,21 @@ __device__ inline uint64_t wg_sum_u64(uint64_t v) {
   return t;
 }
 
+}  // namespace
+
 // Output form of an item (storage hash S = (hash << value_bits) | value):
 //   GQF_ENUM_OUT_HASH  S >> value_bits (qfi_get_hash's key), value to `values`
 //   GQF_ENUM_OUT_KEY   hash_64i of that (qfi_get_key's key)
 //   GQF_ENUM_OUT_MOVE  the storage hash of the same (hash, value) in a filter
 //                      with `dst_vb` value bits (value truncated to them)
-struct out_form {
+struct gqf_out_form {
   int mode;
   int vb;       // source value bits
   int dst_vb;   // MOVE: destination value bits
   uint64_t key_mask;
 };
 
-__device__ inline uint64_t form_hash(const out_form& o, uint64_t s) {
+static __device__ inline uint64_t form_hash(const gqf_out_form& o, uint64_t s) {
   const uint64_t h = s >> o.vb;
   if (o.mode == GQF_ENUM_OUT_KEY) return gqf_hash64_inv(h, o.key_mask);
   if (o.mode == GQF_ENUM_OUT_MOVE) return (h << o.dst_vb) | (s & gqf_mask((uint64_t)min(o.vb, o.dst_vb)));
@@ -144,9 +147,12 @@ __device__ inline uint64_t form_hash(const out_form& o, uint64_t s) {
 
 // Pass 1: items per workgroup; statistics of the whole range into stats[0..2]
 // (items, sum of counts, used slots) and damaged-table flag into status.
-__global__ void __launch_bounds__(kThreads)
+// With `starts` set, also the bit of every item's first slot, in words
+// indexed from block b_lo, and the highest word marked in *max_word.
+extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_enum_count(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-               uint32_t* __restrict__ wg_items, unsigned long long* __restrict__ stats) {
+               uint32_t* __restrict__ wg_items, unsigned long long* __restrict__ stats,
+               unsigned long long* __restrict__ starts, unsigned long long* __restrict__ max_word) {
   __shared__ uint32_t rows[kThreads * 33];
   const QF& f = *qfp;
   const uint64_t b = b_lo + (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -156,10 +162,24 @@ gqf_enum_count(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_
     uint32_t* row = rows + threadIdx.x * 33;
     load_row(f, b, row);
     const slot_row sl{&f, b * 64, (const uint8_t*)row};
-    const int64_t u = walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t, uint64_t c) {
+    uint64_t cw = ~0ULL, cb = 0;  // word being marked and its pending bits
+    const int64_t u = walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t, uint64_t c, uint64_t pos) {
       items++;
       sum += c;
+      if (starts) {
+        const uint64_t wd = (pos >> 6) - b_lo;
+        if (wd != cw) {
+          if (cb) atomicOr(&starts[cw], (unsigned long long)cb);
+          cw = wd;
+          cb = 0;
+        }
+        cb |= 1ULL << (pos & 63);
+      }
     });
+    if (cb) {
+      atomicOr(&starts[cw], (unsigned long long)cb);
+      atomicMax(max_word, (unsigned long long)cw);
+    }
     if (u < 0) atomicOr(f.status, GQF_STATUS_CLUSTER_OVERFLOW);
     else used = (uint64_t)u;
   }
@@ -177,7 +197,7 @@ gqf_enum_count(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_
 
 // Exclusive scan of n u32 counts into u64 bases (one workgroup); the total
 // goes to base[n].
-__global__ void __launch_bounds__(1024) gqf_enum_scan(const uint32_t* __restrict__ cnt, uint32_t n,
+extern "C" __global__ void __launch_bounds__(1024) gqf_enum_scan(const uint32_t* __restrict__ cnt, uint32_t n,
                                                       uint64_t* __restrict__ base) {
   __shared__ uint64_t wsum[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -206,35 +226,87 @@ __global__ void __launch_bounds__(1024) gqf_enum_scan(const uint32_t* __restrict
   if (threadIdx.x == 0) base[n] = carry;
 }
 
-// Pass 2: every item to its rank (entries at ranks >= cap are dropped).
-__global__ void __launch_bounds__(kThreads)
-gqf_enum_write(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-               const uint64_t* __restrict__ wg_base, uint64_t r0, out_form o, uint64_t* __restrict__ hashes,
-               uint64_t* __restrict__ counts, uint64_t* __restrict__ values, uint64_t cap) {
-  __shared__ uint32_t rows[kThreads * 33];
-  const QF& f = *qfp;
-  const uint64_t b = b_lo + (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const bool mine = b < b_hi;
-  uint32_t* row = rows + threadIdx.x * 33;
-  uint32_t items = 0;
-  if (mine) {
-    load_row(f, b, row);
-    const slot_row sl{&f, b * 64, (const uint8_t*)row};
-    walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t, uint64_t) { items++; });
-  }
+// Pass 2a: per word of marks, the number of marks in earlier words of its
+// 256-word group (local_prefix) and each group's total (group_sums).
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_enum_words(const unsigned long long* __restrict__ starts, uint64_t nwords, uint32_t* __restrict__ local_prefix,
+               uint32_t* __restrict__ group_sums) {
+  const uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t c = w < nwords ? (uint32_t)__builtin_popcountll(starts[w]) : 0;
   uint32_t tot;
-  uint64_t r = wg_base[blockIdx.x] - r0 + wg_exclusive_scan(items, &tot);
-  if (!mine || items == 0 || r >= cap) return;
-  const slot_row sl{&f, b * 64, (const uint8_t*)row};
-  const uint64_t vmask = gqf_mask((uint64_t)o.vb);
-  walk_items(f, b, q_lo, q_hi, sl, [&](uint64_t s, uint64_t c) {
-    if (r < cap) {
-      if (hashes) hashes[r] = form_hash(o, s);
-      if (counts) counts[r] = c;
-      if (values) values[r] = s & vmask;
-    }
-    r++;
-  });
+  const uint32_t before = wg_exclusive_scan(c, &tot);
+  if (w < nwords) local_prefix[w] = before;
+  if (threadIdx.x == 0) group_sums[blockIdx.x] = tot;
+}
+
+// Quotient whose run holds used slot p.  Past block b's offset the runs of
+// quotients below 64 b have ended, so it is block b's k-th occupied
+// quotient, k = run ends of block b in [offset, p).  Inside the offset the
+// slot belongs to an earlier block b2: the last one whose own runs start at
+// or before p, and k counts the run ends in [64 b2 + offset(b2), p).
+// Returns ~0 for a table whose bits do not pair up.
+static __device__ inline uint64_t quotient_of_slot(const QF& f, uint64_t p) {
+  const uint64_t b = p >> 6;
+  const uint64_t i = p & 63;
+  const uint64_t off = gqf_point::offset(f, b);
+  if (i >= off) {
+    const int k = __builtin_popcountll(gqf_point::run_word(f, b) & gqf_mask(i) & ~gqf_mask(off));
+    const int sel = gqf_select64(gqf_point::occ_word(f, b), k);
+    return sel < 64 ? b * 64 + (uint64_t)sel : ~0ULL;
+  }
+  uint64_t b2 = b - 1;  // b > 0: block 0 has offset 0
+  while (b2 > 0 && b2 * 64 + gqf_point::offset(f, b2) > p) b2--;
+  const uint64_t x = b2 * 64 + gqf_point::offset(f, b2);
+  uint64_t k = 0;
+  for (uint64_t j = x >> 6; j <= b; j++) {
+    uint64_t wd = gqf_point::run_word(f, j);
+    if (j == (x >> 6)) wd &= ~gqf_mask(x & 63);
+    if (j == b) wd &= gqf_mask(i);
+    k += (uint64_t)__builtin_popcountll(wd);
+  }
+  for (uint64_t j = b2; j < b; j++) {
+    const uint64_t o = gqf_point::occ_word(f, j);
+    const uint64_t c = (uint64_t)__builtin_popcountll(o);
+    if (k < c) return j * 64 + (uint64_t)gqf_select64(o, (int)k);
+    k -= c;
+  }
+  return ~0ULL;
+}
+
+// Pass 2b: one lane per slot of the marked words (wave per word): a lane
+// whose slot starts a counter decodes it and writes the item at its rank,
+// so a wave's items go out as one contiguous stretch.  Each word is cleared
+// after it is read, leaving the marks zero for the next use.
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_enum_emit(const QF* __restrict__ qfp, uint64_t w_lo, uint64_t nwords,
+              const unsigned long long* __restrict__ max_word, unsigned long long* __restrict__ starts,
+              const uint32_t* __restrict__ local_prefix, const uint64_t* __restrict__ group_base, gqf_out_form o,
+              uint64_t* __restrict__ hashes, uint64_t* __restrict__ counts, uint64_t* __restrict__ values,
+              uint64_t cap) {
+  const QF& f = *qfp;
+  const uint64_t rel = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const uint64_t i = threadIdx.x & 63;
+  if (rel >= nwords || rel > *max_word) return;
+  const uint64_t bits = starts[rel];
+  if (!bits) return;
+  if (i == 0) starts[rel] = 0;
+  if (!((bits >> i) & 1)) return;
+  const uint64_t rank = group_base[rel / kThreads] + local_prefix[rel] + (uint64_t)__builtin_popcountll(bits & gqf_mask(i));
+  if (rank >= cap) return;
+  const uint64_t p = (w_lo + rel) * 64 + i;
+  const uint64_t q = quotient_of_slot(f, p);
+  const int64_t e = gqf_point::runend_from(f, p, 0);
+  if (q == ~0ULL || e < (int64_t)p) {
+    atomicOr(f.status, GQF_STATUS_CLUSTER_OVERFLOW);
+    return;
+  }
+  auto at = [&](uint32_t k) -> uint64_t { return gqf_point::slot(f, p + k); };
+  uint64_t rem, cnt;
+  gqf_decode_counter(at, (uint32_t)(e - (int64_t)p + 1), 0, &rem, &cnt);
+  const uint64_t sh = (q << 8) | rem;
+  if (hashes) hashes[rank] = form_hash(o, sh);
+  if (counts) counts[rank] = cnt;
+  if (values) values[rank] = sh & gqf_mask((uint64_t)o.vb);
 }
 
 // ---------------------------------------------------------------------------
@@ -244,7 +316,7 @@ gqf_enum_write(const QF* __restrict__ qfp, uint64_t b_lo, uint64_t b_hi, uint64_
 // used iff the runs of earlier blocks reach it (i < offset) or some quotient
 // of block b at or below it has not ended its run before it
 // (offset_lower_bound != 0, src/gqf.cu:681-705).
-__device__ inline uint64_t used_mask(const QF& f, uint64_t b) {
+static __device__ inline uint64_t used_mask(const QF& f, uint64_t b) {
   const gqf_header h = gqf_load_header(f, b);
   uint64_t used;
   if (h.offset >= 64) {
@@ -267,11 +339,11 @@ __device__ inline uint64_t used_mask(const QF& f, uint64_t b) {
 
 // whether slot 64 b - 1 is used: earlier runs reach into block b, or a run
 // ends exactly there
-__device__ inline uint64_t prev_slot_used(const QF& f, uint64_t b) {
+static __device__ inline uint64_t prev_slot_used(const QF& f, uint64_t b) {
   return b > 0 && (gqf_point::offset(f, b) > 0 || (gqf_load_runends(f, b - 1) >> 63)) ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kThreads)
+extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_census_mask(const QF* __restrict__ qfp, uint64_t* __restrict__ used, uint32_t* __restrict__ wg_starts) {
   const QF& f = *qfp;
   const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -287,7 +359,7 @@ gqf_census_mask(const QF* __restrict__ qfp, uint64_t* __restrict__ used, uint32_
   if (threadIdx.x == 0) wg_starts[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kThreads)
+extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_census_write(const QF* __restrict__ qfp, const uint64_t* __restrict__ used, const uint64_t* __restrict__ wg_base,
                  uint64_t* __restrict__ lengths, uint64_t cap, unsigned long long* __restrict__ hist, uint32_t nbins,
                  unsigned long long* __restrict__ max_len) {
@@ -326,27 +398,37 @@ gqf_census_write(const QF* __restrict__ qfp, const uint64_t* __restrict__ used, 
   if (my_max) atomicMax(max_len, (unsigned long long)my_max);
 }
 
-}  // namespace
-
 uint64_t gqf_enum_wg_count(uint64_t nblocks) { return (nblocks + kThreads - 1) / kThreads; }
 
 void gqf_launch_enum_count(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats, hipStream_t s) {
+                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats,
+                           unsigned long long* starts, unsigned long long* max_word, hipStream_t s) {
   const uint64_t nwg = gqf_enum_wg_count(b_hi - b_lo);
   if (nwg == 0) return;
+  gqf_prof_begin("enum_count", s);
   hipLaunchKernelGGL(gqf_enum_count, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, b_lo, b_hi, q_lo, q_hi,
-                     wg_items, stats);
+                     wg_items, stats, starts, max_word);
+  gqf_prof_end("enum_count", s);
   hipLaunchKernelGGL(gqf_enum_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)wg_items, (uint32_t)nwg, wg_base);
 }
 
-void gqf_launch_enum_write(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-                           const uint64_t* wg_base, uint64_t r0, int mode, int vb, int dst_vb, uint64_t key_mask,
-                           uint64_t* hashes, uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s) {
-  const uint64_t nwg = gqf_enum_wg_count(b_hi - b_lo);
-  if (nwg == 0 || cap == 0) return;
-  const out_form o{mode, vb, dst_vb, key_mask};
-  hipLaunchKernelGGL(gqf_enum_write, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, b_lo, b_hi, q_lo, q_hi,
-                     wg_base, r0, o, hashes, counts, values, cap);
+void gqf_launch_enum_emit(const QF* qf_dev, uint64_t w_lo, uint64_t nwords, unsigned long long* starts,
+                          const unsigned long long* max_word, uint32_t* local_prefix, uint32_t* group_sums,
+                          uint64_t* group_base, int mode, int vb, int dst_vb, uint64_t key_mask, uint64_t* hashes,
+                          uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s) {
+  const uint64_t ngroups = gqf_enum_wg_count(nwords);
+  if (ngroups == 0) return;
+  hipLaunchKernelGGL(gqf_enum_words, dim3((uint32_t)ngroups), dim3(kThreads), 0, s,
+                     (const unsigned long long*)starts, nwords, local_prefix, group_sums);
+  hipLaunchKernelGGL(gqf_enum_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)group_sums, (uint32_t)ngroups,
+                     group_base);
+  const gqf_out_form o{mode, vb, dst_vb, key_mask};
+  const uint64_t wgs = (nwords + kThreads / 64 - 1) / (kThreads / 64);
+  gqf_prof_begin("enum_emit", s);
+  hipLaunchKernelGGL(gqf_enum_emit, dim3((uint32_t)wgs), dim3(kThreads), 0, s, qf_dev, w_lo, nwords, max_word,
+                     starts, (const uint32_t*)local_prefix, (const uint64_t*)group_base, o, hashes, counts, values,
+                     cap);
+  gqf_prof_end("enum_emit", s);
 }
 
 void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint32_t* wg_starts, uint64_t* wg_base,
@@ -354,8 +436,12 @@ void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint3
                        unsigned long long* max_len, hipStream_t s) {
   const uint64_t nwg = gqf_enum_wg_count(nblocks);
   if (nwg == 0) return;
+  gqf_prof_begin("census_mask", s);
   hipLaunchKernelGGL(gqf_census_mask, dim3((uint32_t)nwg), dim3(kThreads), 0, s, qf_dev, used, wg_starts);
+  gqf_prof_end("census_mask", s);
   hipLaunchKernelGGL(gqf_enum_scan, dim3(1), dim3(1024), 0, s, (const uint32_t*)wg_starts, (uint32_t)nwg, wg_base);
+  gqf_prof_begin("census_write", s);
   hipLaunchKernelGGL(gqf_census_write, dim3((uint32_t)nwg), dim3(kThreads), nbins * sizeof(uint32_t), s, qf_dev,
                      (const uint64_t*)used, (const uint64_t*)wg_base, lengths, cap, hist, nbins, max_len);
+  gqf_prof_end("census_write", s);
 }

@@ -59,15 +59,21 @@ uint64_t gqf_enum_wg_count(uint64_t nblocks);
 // count pass over blocks [b_lo, b_hi), items of quotients [q_lo, q_hi):
 // items per workgroup -> wg_items, their exclusive prefix sums -> wg_base
 // (nwg + 1 entries, the last the total); stats[0..2] += items, sum of
-// counts, slots used by the runs
+// counts, slots used by the runs.  With `starts` (zeroed, one word per
+// block from b_lo to the end of the table) it also sets the bit of every
+// item's first slot and raises *max_word to the last word it marked.
 void gqf_launch_enum_count(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats, hipStream_t s);
-// write pass over the same range: item of rank r (wg_base[wg] - r0 + rank
-// in the workgroup) -> hashes[r] / counts[r] / values[r] when r < cap.
-// MOVE writes the slot hash the item has in a filter with dst_vb value bits.
-void gqf_launch_enum_write(const QF* qf_dev, uint64_t b_lo, uint64_t b_hi, uint64_t q_lo, uint64_t q_hi,
-                           const uint64_t* wg_base, uint64_t r0, int mode, int vb, int dst_vb, uint64_t key_mask,
-                           uint64_t* hashes, uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s);
+                           uint32_t* wg_items, uint64_t* wg_base, unsigned long long* stats,
+                           unsigned long long* starts, unsigned long long* max_word, hipStream_t s);
+// write pass over those marks (words w_lo .. w_lo + nwords): the item whose
+// counter starts at the r-th mark -> hashes[r] / counts[r] / values[r] when
+// r < cap, in the form `mode` (MOVE: the slot hash the item has in a filter
+// with dst_vb value bits); clears the marks.  Scratch: local_prefix nwords
+// words, group_sums and group_base (+1) one per 256 words.
+void gqf_launch_enum_emit(const QF* qf_dev, uint64_t w_lo, uint64_t nwords, unsigned long long* starts,
+                          const unsigned long long* max_word, uint32_t* local_prefix, uint32_t* group_sums,
+                          uint64_t* group_base, int mode, int vb, int dst_vb, uint64_t key_mask, uint64_t* hashes,
+                          uint64_t* counts, uint64_t* values, uint64_t cap, hipStream_t s);
 // cluster census: used-slot masks (nblocks words), cluster starts per
 // workgroup and their bases (nwg + 1, the last = number of clusters);
 // lengths[rank] (rank < cap), hist[min(len, nbins - 1)] += 1 (nbins <= 4096,

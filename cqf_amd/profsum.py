@@ -1,11 +1,14 @@
 """Summaries of rocprofv3 CSV output (kernel stats and counter collection).
 
     python -m cqf_amd.profsum stats  <run>_kernel_stats.csv
-    python -m cqf_amd.profsum counters <run>_counter_collection.csv
+    python -m cqf_amd.profsum counters <run>_counter_collection.csv [--by-grid]
+    python -m cqf_amd.profsum trace <run>_kernel_trace.csv
 
 `stats` prints calls / total / average duration per kernel; `counters`
 prints, per kernel and counter, the number of dispatches and the mean value
-per dispatch (FETCH_SIZE and WRITE_SIZE are in KiB).
+per dispatch (FETCH_SIZE and WRITE_SIZE are in KiB); `trace` and
+`counters --by-grid` split each kernel by grid size, which separates
+launches of one kernel over different amounts of work.
 """
 import csv
 import sys
@@ -27,17 +30,32 @@ def stats(path):
               f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
 
 
-def counters(path):
+def counters(path, by_grid=False):
     acc = defaultdict(lambda: [0, 0.0])
     for r in csv.DictReader(open(path)):
-        k = (r["Kernel_Name"].strip(), r["Counter_Name"].strip())
+        k = (r["Kernel_Name"].strip(), r["Counter_Name"].strip(), r["Grid_Size"] if by_grid else "")
         acc[k][0] += 1
         acc[k][1] += float(r["Counter_Value"])
-    print("| kernel | counter | dispatches | mean per dispatch |")
-    print("|---|---|---|---|")
-    for (kern, ctr), (n, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
-        print(f"| {_short(kern)} | {ctr} | {n} | {tot / n:.1f} |")
+    print("| kernel | grid | counter | dispatches | mean per dispatch |")
+    print("|---|---|---|---|---|")
+    for (kern, ctr, grid), (n, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {_short(kern)} | {grid or '-'} | {ctr} | {n} | {tot / n:.1f} |")
+
+
+def trace(path):
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        acc[(r["Kernel_Name"].strip(), r["Grid_Size_X"])].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    print("| kernel | grid | calls | avg us | min us | max us |")
+    print("|---|---|---|---|---|---|")
+    for (kern, grid), d in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| {_short(kern)} | {grid} | {len(d)} | {sum(d) / len(d):.2f} | {min(d):.2f} | {max(d):.2f} |")
 
 
 if __name__ == "__main__":
-    {"stats": stats, "counters": counters}[sys.argv[1]](sys.argv[2])
+    cmd, path = sys.argv[1], sys.argv[2]
+    if cmd == "counters":
+        counters(path, by_grid="--by-grid" in sys.argv[3:])
+    else:
+        {"stats": stats, "trace": trace}[cmd](path)

@@ -232,7 +232,8 @@ int qf_last_error(void);
 
 /* per-kernel hipEvent timing on the launch stream: enable (clears), then
  * read the summed milliseconds and launch count of one kernel name:
- * "sort_pass", "region_bounds", "rebuild_phase", "rebuild_wide", "bulk_get" */
+ * "sort_pass", "region_bounds", "rebuild_phase", "rebuild_wide", "bulk_get",
+ * "enum_count", "enum_emit", "census_mask", "census_write" */
 void gqf_profile_enable(int on);
 int gqf_profile_read(const char *name, double *total_ms, uint64_t *launches);
 /* dev_out[i] = splitmix64 output number skip+i of the stream seeded with seed */
