@@ -152,8 +152,8 @@ __host__ __device__ static inline uint64_t gqf_hash64_inv(uint64_t h, uint64_t m
 }
 
 // Slot hash of a key: the reference's (hash_64(key) << value_bits) | value
-// with value 0 (qf_insert / qf_count_key_value, src/gqf.cu:2475-2480,
-// 3484-3492); quotient = slot hash >> 8, remainder = its low 8 bits.
+// with value 0 (qf_insert / qf_count_key_value, src/gqf.cu:2486,
+// 3492); quotient = slot hash >> 8, remainder = its low 8 bits.
 // value_bits is 0 for every filter qf_malloc_device makes; a filter grown by
 // qf_resize_device keeps its hash width (key_bits) and gains value bits, so
 // its remainders carry that many zero bits below the hash's.
