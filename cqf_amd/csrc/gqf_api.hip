@@ -145,6 +145,17 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
   gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
                   w.hist, w.totals, s);
+  // A sparse insert batch -- at most one key per point-lock region and one
+  // per two rebuild regions on average -- would make the rebuild rewrite a
+  // whole window for about one key each; the point path inserts each key
+  // where it belongs instead (the image is the same: it depends on the
+  // multiset only).  Hashed keys of a filter without value bits, so the
+  // sorted keys are the hashes point_insert takes.
+  if (mode == kModeInsert && hash_it && vb == 0 && n <= f->host.nregions / 2 && n <= f->host.num_locks) {
+    gqf_launch_sparse_insert(f->dev, keys, n, s);
+    GQF_CHECK(hipGetLastError());
+    return;
+  }
   const uint64_t* items = keys;
   const uint64_t* counts = nullptr;
   const uint64_t* n_dev = nullptr;

@@ -43,6 +43,9 @@ void gqf_launch_export(const QF* qf_dev, uint64_t nblocks, uint8_t* out, hipStre
 // point API drivers: one lane per key (rets / counts / misses may be null)
 void gqf_launch_point_insert(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_t flags, int32_t* rets,
                              int not_exists, hipStream_t s);
+// bulk insert of hashed keys one lane per key through point_insert (the
+// region locks); full table -> QF_STATUS_NO_SPACE
+void gqf_launch_sparse_insert(QF* qf_dev, const uint64_t* hashes, uint64_t n, hipStream_t s);
 void gqf_launch_point_query(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_t flags, uint64_t* counts,
                             int concurrent, unsigned long long* misses, hipStream_t s);
 

@@ -54,6 +54,22 @@ gqf_point_query_kernel(QF* qf, const uint64_t* __restrict__ keys, uint64_t n, ui
   }
 }
 
+// Sparse bulk insert: point_insert of each (already hashed) key; a key the
+// table has no room for sets QF_STATUS_NO_SPACE as a bulk call would.
+extern "C" __global__ void __launch_bounds__(256)
+gqf_sparse_insert_kernel(QF* qf, const uint64_t* __restrict__ hashes, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (point_insert(qf, hashes[i], 0, QF_KEY_IS_HASH) == QF_FULL) atomicOr(qf->status, GQF_STATUS_NO_SPACE);
+  }
+}
+
+void gqf_launch_sparse_insert(QF* qf_dev, const uint64_t* hashes, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  gqf_prof_begin("sparse_insert", s);
+  hipLaunchKernelGGL(gqf_sparse_insert_kernel, dim3(point_grid(n)), dim3(256), 0, s, qf_dev, hashes, n);
+  gqf_prof_end("sparse_insert", s);
+}
+
 void gqf_launch_point_insert(QF* qf_dev, const uint64_t* keys, uint64_t n, uint8_t flags, int32_t* rets,
                              int not_exists, hipStream_t s) {
   if (n == 0) return;

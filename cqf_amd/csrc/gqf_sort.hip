@@ -18,6 +18,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;  // 4096 keys
+constexpr int kItemsSmall = 4;            // tiles of 1024 keys for inputs up to kSmallN
+constexpr uint64_t kSmallN = 1ULL << 16;
+constexpr uint32_t kScanSmallTiles = 64;  // one-workgroup scan up to this many tiles
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRadix = 1024;           // digits of up to 10 bits
 
@@ -35,18 +38,19 @@ __device__ inline uint64_t load_key(const uint64_t* in, uint64_t i, uint64_t n, 
 }  // namespace
 
 // per-tile digit histogram -> hist[d * ntiles + tile]
-extern "C" __global__ void __launch_bounds__(kThreads)
-gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits, int hash_it,
-                 uint64_t mask, int vshift, uint32_t* __restrict__ hist, uint32_t ntiles) {
+template <int ITEMS>
+__device__ __forceinline__ void upsweep_body(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits,
+                                             int hash_it, uint64_t mask, int vshift, uint32_t* __restrict__ hist,
+                                             uint32_t ntiles) {
   __shared__ uint32_t h[kMaxRadix];
   const int radix = 1 << bits;
   for (int d = threadIdx.x; d < radix; d += kThreads) h[d] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  const uint64_t base = (uint64_t)blockIdx.x * (kThreads * ITEMS);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll 4
-  for (int k = 0; k < kItems; k++) {
-    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+  for (int k = 0; k < ITEMS; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * ITEMS) + (uint64_t)k * 64 + lane;
     if (i < n) {
       uint64_t key = load_key(in, i, n, hash_it, mask, vshift);
       atomicAdd(&h[(uint32_t)(key >> shift) & (radix - 1)], 1u);
@@ -54,6 +58,18 @@ gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bit
   }
   __syncthreads();
   for (int d = threadIdx.x; d < radix; d += kThreads) hist[(uint64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits, int hash_it,
+                 uint64_t mask, int vshift, uint32_t* __restrict__ hist, uint32_t ntiles) {
+  upsweep_body<kItems>(in, n, shift, bits, hash_it, mask, vshift, hist, ntiles);
+}
+
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_upsweep_small(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits, int hash_it,
+                       uint64_t mask, int vshift, uint32_t* __restrict__ hist, uint32_t ntiles) {
+  upsweep_body<kItemsSmall>(in, n, shift, bits, hash_it, mask, vshift, hist, ntiles);
 }
 
 // Exclusive scan of each digit's row of tile counts; row totals -> totals[d].
@@ -86,29 +102,81 @@ gqf_sort_scan_rows(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __res
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Exclusive scan in place of count (<= 4 * kThreads) values in LDS by one
+// workgroup: each thread takes four consecutive entries, the threads' sums
+// are scanned by wave shuffles and across the four waves.
+__device__ __forceinline__ void block_exclusive_scan(uint32_t* s, int count) {
+  __shared__ uint32_t wsum[kWaves];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t v[4];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int d = 4 * t + k;
+    v[k] = d < count ? s[d] : 0;
+    sum += v[k];
+  }
+  uint32_t incl = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t x = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += x;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (int k = 0; k < w; k++) run += wsum[k];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int d = 4 * t + k;
+    if (d < count) s[d] = run;
+    run += v[k];
+  }
+  __syncthreads();
+}
+
 // Exclusive scan of the digit totals (one workgroup, radix <= 1024).
 extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_sort_scan_totals(uint32_t* __restrict__ totals, int radix) {
   __shared__ uint32_t s[kMaxRadix];
   for (int d = threadIdx.x; d < radix; d += kThreads) s[d] = totals[d];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  block_exclusive_scan(s, radix);
+  for (int d = threadIdx.x; d < radix; d += kThreads) totals[d] = s[d];
+}
+
+// Both scans for up to kScanSmallTiles tiles in one workgroup: each digit's
+// row of tile counts becomes its exclusive prefix (as gqf_sort_scan_rows),
+// and the digit totals their exclusive scan (as gqf_sort_scan_totals).
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_scan_small(uint32_t* __restrict__ hist, uint32_t ntiles, int radix, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s[kMaxRadix];
+  for (int d = threadIdx.x; d < radix; d += kThreads) {
+    uint32_t* row = hist + (uint64_t)d * ntiles;
     uint32_t run = 0;
-    for (int d = 0; d < radix; d++) { uint32_t t = s[d]; s[d] = run; run += t; }
+    for (uint32_t i = 0; i < ntiles; i++) {
+      const uint32_t v = row[i];
+      row[i] = run;
+      run += v;
+    }
+    s[d] = run;
   }
   __syncthreads();
+  block_exclusive_scan(s, radix);
   for (int d = threadIdx.x; d < radix; d += kThreads) totals[d] = s[d];
 }
 
 // Stable scatter of one tile.
-extern "C" __global__ void __launch_bounds__(kThreads)
-gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int shift,
-                   int bits, int hash_it, uint64_t mask, int vshift, const uint32_t* __restrict__ hist,
-                   const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+template <int ITEMS>
+__device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                               uint64_t n, int shift, int bits, int hash_it, uint64_t mask,
+                                               int vshift, const uint32_t* __restrict__ hist,
+                                               const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+  constexpr int TILE = kThreads * ITEMS;
   __shared__ uint32_t wcnt[kWaves][kMaxRadix];
   __shared__ uint32_t tile_start[kMaxRadix];
   __shared__ uint32_t glob[kMaxRadix];
-  __shared__ uint64_t stage[kTile];
+  __shared__ uint64_t stage[TILE];
   const int radix = 1 << bits;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < radix; d += kThreads) {
@@ -118,13 +186,13 @@ gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, 
   }
   __syncthreads();
 
-  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  const uint64_t base = (uint64_t)blockIdx.x * TILE;
   const uint64_t lt_mask = lane ? (~0ULL >> (64 - lane)) : 0ULL;
-  uint64_t keys[kItems];
-  uint32_t rank[kItems];
+  uint64_t keys[ITEMS];
+  uint32_t rank[ITEMS];
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+  for (int k = 0; k < ITEMS; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * ITEMS) + (uint64_t)k * 64 + lane;
     bool valid = i < n;
     uint64_t key = load_key(in, i, n, hash_it, mask, vshift);
     uint32_t d = (uint32_t)(key >> shift) & (radix - 1);
@@ -170,20 +238,34 @@ gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, 
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    uint64_t i = base + (uint64_t)w * (64 * kItems) + (uint64_t)k * 64 + lane;
+  for (int k = 0; k < ITEMS; k++) {
+    uint64_t i = base + (uint64_t)w * (64 * ITEMS) + (uint64_t)k * 64 + lane;
     if (i < n) {
       uint32_t d = (uint32_t)(keys[k] >> shift) & (radix - 1);
       stage[tile_start[d] + wcnt[w][d] + rank[k]] = keys[k];
     }
   }
   __syncthreads();
-  const uint32_t cnt = (uint32_t)min((uint64_t)kTile, n - base);
+  const uint32_t cnt = (uint32_t)min((uint64_t)TILE, n - base);
   for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
     uint64_t key = stage[i];
     uint32_t d = (uint32_t)(key >> shift) & (radix - 1);
     out[(uint64_t)glob[d] + (i - tile_start[d])] = key;
   }
+}
+
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int shift,
+                   int bits, int hash_it, uint64_t mask, int vshift, const uint32_t* __restrict__ hist,
+                   const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+  downsweep_body<kItems>(in, out, n, shift, bits, hash_it, mask, vshift, hist, bin_base, ntiles);
+}
+
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_sort_downsweep_small(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int shift,
+                         int bits, int hash_it, uint64_t mask, int vshift, const uint32_t* __restrict__ hist,
+                         const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+  downsweep_body<kItemsSmall>(in, out, n, shift, bits, hash_it, mask, vshift, hist, bin_base, ntiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,7 +361,11 @@ extern "C" __global__ void gqf_rle_counts(const uint64_t* __restrict__ upos, con
 
 static inline uint32_t sort_ntiles(uint64_t n) { return (uint32_t)((n + kTile - 1) / kTile); }
 
-uint64_t gqf_sort_hist_words(uint64_t n) { return (uint64_t)kMaxRadix * (sort_ntiles(n) ? sort_ntiles(n) : 1); }
+// sized for the larger of the 4096-key tiling and the small-input tiling
+uint64_t gqf_sort_hist_words(uint64_t n) {
+  const uint64_t t = sort_ntiles(n);
+  return (uint64_t)kMaxRadix * (t > kScanSmallTiles ? t : kScanSmallTiles);
+}
 
 void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int vshift, int bits,
                      int digit_bits, uint32_t* hist, uint32_t* totals, hipStream_t s) {
@@ -298,7 +384,10 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
     passes = 8;
     digit = 8;
   }
-  const uint32_t ntiles = sort_ntiles(n);
+  // small inputs: 1024-key tiles (more workgroups, shorter tiles)
+  const bool small = n <= kSmallN;
+  const uint32_t tile = small ? kThreads * kItemsSmall : kTile;
+  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
   const int radix = 1 << digit;
   uint64_t* in = keys;
   uint64_t* out = tmp;
@@ -306,12 +395,17 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
     const int shift = p * digit;
     const int h = p == 0 ? hash_it : 0;
     gqf_prof_begin("sort_pass", s);
-    hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n, shift, digit, h, mask, vshift,
-                       hist, ntiles);
-    hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
-    hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
-    hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)in, out, n, shift,
-                       digit, h, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+    hipLaunchKernelGGL(small ? gqf_sort_upsweep_small : gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, in, n,
+                       shift, digit, h, mask, vshift, hist, ntiles);
+    if (ntiles <= kScanSmallTiles) {
+      hipLaunchKernelGGL(gqf_sort_scan_small, dim3(1), dim3(kThreads), 0, s, hist, ntiles, radix, totals);
+    } else {
+      hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
+      hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
+    }
+    hipLaunchKernelGGL(small ? gqf_sort_downsweep_small : gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s,
+                       (const uint64_t*)in, out, n, shift, digit, h, mask, vshift, (const uint32_t*)hist,
+                       (const uint32_t*)totals, ntiles);
     gqf_prof_end("sort_pass", s);
     uint64_t* t = in;
     in = out;

@@ -217,3 +217,35 @@ def test_tuning_does_not_change_results(gpu, oracle, region_blocks, digit):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("nbits", [20, 22])
+def test_sparse_batches_after_dense_fill(gpu, oracle, nbits):
+    """Batches of at most one key per point-lock region take the point path
+    (one lane per key under the region locks); the image must equal the
+    oracle's after each, including keys repeated within a batch and keys
+    already present, through bulk_insert and bulk_insert_reduce."""
+    from cqf_amd.gqf import to_device_u64
+    from cqf_amd.gqf import GQF
+    _say(f"case sparse batches q={nbits}")
+    keys = oracle.splitmix64(31, int(0.85 * (1 << nbits)))
+    g, o = _gpu_and_oracle(oracle, nbits)
+    try:
+        g.bulk_insert(to_device_u64(keys))
+        o.bulk_insert(keys.copy())
+        limit = min(g.info["nregions"] // 2, g.info["num_locks"])
+        extra = oracle.splitmix64(32, 40 * limit)
+        for b in range(20):
+            part = extra[b * limit:(b + 1) * limit - 3]
+            part = np.concatenate([part, part[:2], keys[b:b + 1]])[:limit]  # repeats and a present key
+            if b % 2:
+                g.bulk_insert(to_device_u64(part))
+            else:
+                g.bulk_insert_reduce(to_device_u64(part))
+            o.bulk_insert(part.copy())
+        gpu.cuda.synchronize()
+        _assert_images_equal(g.export_image(), o.image(), "image after sparse batches")
+        assert g.status() == 0
+    finally:
+        g.close()
+        o.close()
