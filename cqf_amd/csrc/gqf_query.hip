@@ -97,7 +97,10 @@ gqf_bulk_get_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ val
   uint32_t* row = rows + 17 * threadIdx.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    returns[i] = gqf_count_hash(qf, gqf_key_hash(qf, vals[i]), row);
+    // keys and results stream through once: non-temporal, so they do not
+    // displace table lines from the caches
+    const uint64_t key = __builtin_nontemporal_load(&vals[i]);
+    __builtin_nontemporal_store(gqf_count_hash(qf, gqf_key_hash(qf, key), row), &returns[i]);
   }
 }
 
@@ -111,7 +114,7 @@ gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
   uint32_t mine = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    mine += gqf_count_hash(qf, gqf_key_hash(qf, vals[i]), row) < 1;
+    mine += gqf_count_hash(qf, gqf_key_hash(qf, __builtin_nontemporal_load(&vals[i])), row) < 1;
   }
   // wave reduction, one atomic per wave
   for (int d = 32; d >= 1; d >>= 1) mine += __shfl_down(mine, d, 64);
