@@ -53,9 +53,8 @@ struct Smem {
   int32_t xnew[W + 1];
   uint64_t occ_in[W];
   uint32_t items[kItemCap];  // hash - first hash of the region
-  uint32_t icnt[kItemCap];   // counts (reduce mode)
-  // one entry per distinct quotient among the items (fast path)
-  uint16_t tq[kItemCap];     // window-relative quotient
+  // one entry per distinct quotient among the items (fast path); the
+  // quotient itself is items[tit0[t]] >> 8
   uint16_t tit0[kItemCap];   // first item of the quotient
   int16_t tst[kItemCap];     // old run start
   uint16_t tol[kItemCap];    // old run length (0: quotient was empty)
@@ -86,19 +85,22 @@ __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
 
 
 // The region's items: cached relative hashes in LDS, or (when the region
-// holds more than kItemCap) read from global memory.
+// holds more than kItemCap) read from global memory.  Counts (reduce mode)
+// are cached in a separate LDS array that only the counting kernels
+// declare, so the plain insert and delete kernels keep their LDS small.
 struct Items {
   const uint64_t* __restrict__ g;
   const uint64_t* __restrict__ gc;
+  uint32_t* lcnt;  // LDS cache of the counts (cached and gc != nullptr)
   uint64_t s;     // first item index of the region
   uint64_t base;  // first hash of the region
   bool cached;
   __device__ inline uint32_t hash(const uint32_t* lds, uint32_t i) const {
     return cached ? lds[i] : (uint32_t)(g[s + i] - base);
   }
-  __device__ inline uint64_t count(const uint32_t* lds, uint32_t i) const {
+  __device__ inline uint64_t count(uint32_t i) const {
     if (!gc) return 1;
-    return cached ? (uint64_t)lds[i] : gc[s + i];
+    return cached ? (uint64_t)lcnt[i] : gc[s + i];
   }
 };
 
@@ -135,7 +137,7 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
     n_rem = h0 & 0xFF;
     n_cnt = 0;
     while (it < it1 && src.hash(sm.items, it) == h0) {
-      n_cnt += src.count(sm.icnt, it);
+      n_cnt += src.count(it);
       it++;
     }
   };
@@ -372,7 +374,7 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
 template <int W>
 __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
-                              Smem<W>& sm, uint64_t tag = 0) {
+                              Smem<W>& sm, uint32_t* lcnt, uint64_t tag = 0) {
   constexpr int kRounds = W / 64;
   const int lane = lane_id();
   const int R = (int)qf.region_blocks;
@@ -382,7 +384,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   const int64_t win_base = (int64_t)(B0 * 64);
   const bool table_end_in_window = B0 + (uint64_t)nwin >= qf.nblocks;
   const uint32_t m = (uint32_t)(e - s);
-  const Items src{items, counts, s, tag + (B0 << 14), m <= (uint32_t)kItemCap};
+  const Items src{items, counts, lcnt, s, tag + (B0 << 14), m <= (uint32_t)kItemCap};
 
   // ---- 1. stage the window and the items ----
   // lane j reads block j's line: header into registers, slots into LDS row j
@@ -407,7 +409,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     if (src.cached) {
       for (uint32_t i = lane; i < m; i += 64) {
         sm.items[i] = (uint32_t)(items[s + i] - src.base);
-        if (counts) sm.icnt[i] = (uint32_t)counts[s + i];
+        if (counts) lcnt[i] = (uint32_t)counts[s + i];
       }
     }
     sm.blk_cnt[lane] = 0;
@@ -449,7 +451,6 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       const uint64_t hm = __ballot(head);
       if (head) {
         const uint32_t t = carry + (uint32_t)__popcll(hm & lt);
-        sm.tq[t] = (uint16_t)(h >> 8);
         sm.tit0[t] = (uint16_t)i;
         atomicAdd(&sm.blk_cnt[h >> 14], 1u);
         atomicOr((unsigned long long*)&sm.tmask[h >> 14], 1ULL << ((h >> 8) & 63));
@@ -475,7 +476,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     // old run and new length of every listed quotient, one quotient per lane
     const uint32_t nq = sm.nq;
     for (uint32_t t = lane; t < nq; t += 64) {
-      const int32_t q = sm.tq[t];
+      const int32_t q = (int32_t)(sm.items[sm.tit0[t]] >> 8);
       int32_t st_o;
       uint32_t old_len;
       uint32_t nl = 0;
@@ -660,21 +661,22 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
 // One workgroup per non-empty region of one parity: workgroup i takes
 // region list[i] (the list gqf_region_bounds compacted; count[parity]
 // entries), so a small batch launches only as many workgroups as it has
-// regions with items.
-extern "C" __global__ void __launch_bounds__(64)
-gqf_rebuild_phase(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
-                  const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
-                  const uint32_t* __restrict__ region_list, const uint32_t* __restrict__ list_count,
-                  int parity, int mode, uint8_t* __restrict__ wide_flags,
-                  uint32_t* __restrict__ wide_count) {
-  __shared__ Smem<kWinFast> sm;
+// regions with items.  gqf_rebuild_phase serves insert and delete (one
+// instance per item); gqf_rebuild_phase_counts takes per-item counts.
+__device__ __forceinline__ void rebuild_phase_item(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
+                                                   const uint64_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ region_start,
+                                                   const uint32_t* __restrict__ region_list,
+                                                   const uint32_t* __restrict__ list_count, int parity, int mode,
+                                                   uint8_t* __restrict__ wide_flags, uint32_t* __restrict__ wide_count,
+                                                   Smem<kWinFast>& sm, uint32_t* lcnt) {
   const QF qf = *qfp;
   if (blockIdx.x >= list_count[parity]) return;
   const uint64_t r = region_list[blockIdx.x];
   if (r >= qf.nregions) return;
   const uint64_t s = region_start[r], e = region_start[r + 1];
   if (s == e) return;
-  const int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm);
+  const int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm, lcnt);
   if (lane_id() == 0) {
     if (res == kNeedWide) {
       wide_flags[r] = 1;
@@ -685,6 +687,27 @@ gqf_rebuild_phase(const QF* __restrict__ qfp, const uint64_t* __restrict__ items
   }
 }
 
+extern "C" __global__ void __launch_bounds__(64)
+gqf_rebuild_phase(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
+                  const uint32_t* __restrict__ region_start, const uint32_t* __restrict__ region_list,
+                  const uint32_t* __restrict__ list_count, int parity, int mode, uint8_t* __restrict__ wide_flags,
+                  uint32_t* __restrict__ wide_count) {
+  __shared__ Smem<kWinFast> sm;
+  rebuild_phase_item(qfp, items, nullptr, region_start, region_list, list_count, parity, mode, wide_flags, wide_count,
+                     sm, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+gqf_rebuild_phase_counts(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
+                         const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
+                         const uint32_t* __restrict__ region_list, const uint32_t* __restrict__ list_count,
+                         int parity, int mode, uint8_t* __restrict__ wide_flags, uint32_t* __restrict__ wide_count) {
+  __shared__ Smem<kWinFast> sm;
+  __shared__ uint32_t icnt[kItemCap];
+  rebuild_phase_item(qfp, items, counts, region_start, region_list, list_count, parity, mode, wide_flags, wide_count,
+                     sm, icnt);
+}
+
 // Redoes flagged regions of one parity in ascending order with the wide
 // window.  One workgroup; exits at once when nothing was flagged.
 extern "C" __global__ void __launch_bounds__(64)
@@ -693,6 +716,7 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                  int parity, int mode, uint8_t* __restrict__ wide_flags,
                  uint32_t* __restrict__ wide_count) {
   __shared__ Smem<kWinWide> sm;
+  __shared__ uint32_t icnt[kItemCap];
   if (*(volatile uint32_t*)wide_count == 0) return;
   const QF qf = *qfp;
   const int lane = lane_id();
@@ -705,7 +729,7 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
       m &= m - 1;
       const uint64_t rr = base + 2ULL * l;
       const uint64_t s = region_start[rr], e = region_start[rr + 1];
-      const int res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm);
+      const int res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
       __syncthreads();
       if (lane == 0) {
         wide_flags[rr] = 0;
@@ -862,7 +886,7 @@ gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode)
     uint64_t s, e, tag;
     region_hash_bounds(g, qf, f, r, items, s, e, tag);
     int res = kOk;
-    if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, tag);
+    if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, nullptr, tag);
     __syncthreads();
     if (lane == 0) {
       if (res == kNeedWide) {
@@ -895,7 +919,7 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
         m &= m - 1;
         uint64_t s, e, tag;
         region_hash_bounds(g, qf, f, rr, items, s, e, tag);
-        const int res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, tag);
+        const int res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, nullptr, tag);
         __syncthreads();
         if (lane == 0) {
           g.wide[f0 + rr] = 0;
@@ -934,8 +958,12 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
   const uint64_t stride = per_parity + 1;
   for (int parity = 0; parity < 2; parity++) {
     gqf_prof_begin("rebuild_phase", s);
-    hipLaunchKernelGGL(gqf_rebuild_phase, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start,
-                       region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
+    if (counts)
+      hipLaunchKernelGGL(gqf_rebuild_phase_counts, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start,
+                         region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
+    else
+      hipLaunchKernelGGL(gqf_rebuild_phase, dim3(g), dim3(64), 0, s, qf_dev, items, region_start,
+                         region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
     gqf_prof_end("rebuild_phase", s);
     gqf_prof_begin("rebuild_wide", s);
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(1), dim3(64), 0, s, qf_dev, items, counts, region_start, parity, mode,
@@ -954,7 +982,7 @@ uint32_t gqf_group_rebuild_grid() {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gqf_group_rebuild, 64, 0);
     if (per_cu < 1) per_cu = 1;
-    if (per_cu > 8) per_cu = 8;  // LDS allows 8 (19.5 KB each of 160 KB); stay within it
+    if (per_cu > 10) per_cu = 10;  // LDS allows 10 (16 KB each of 160 KB); stay within it
     grid = (uint32_t)((cus > 0 ? cus : 1) * per_cu);
   }
   return grid;

@@ -236,37 +236,46 @@ __host__ __device__ static inline uint64_t gqf_table_bytes(uint64_t nblocks) {
 // gqf_counter_len / gqf_counter_emit produce exactly the slots that
 // encode_counter (src/gqf.cu:1216-1272) writes, in slot order.
 
-struct gqf_digits {
-  uint8_t d[12];  // most significant first
-  int n;
-  int pad0;       // leading 0 before the digits (r > 0 and top digit >= r)
-};
-
-__host__ __device__ static inline int gqf_counter_digits(uint64_t rem, uint64_t count, gqf_digits* out) {
-  // count > 3 only
-  uint64_t base = rem ? 254 : 255;
+// A count > 3 as digits: their number n and whether a 0 precedes them
+// (r > 0 and top digit >= r).  The digits themselves are recomputed where
+// they are written, least significant first, so no per-lane buffer is
+// needed (a buffer indexed by digit would live in scratch memory).
+__host__ __device__ static inline void gqf_counter_shape(uint64_t rem, uint64_t count, int* n_out, int* pad0_out) {
+  const uint64_t base = rem ? 254 : 255;
   uint64_t c = rem ? count - 3 : count - 4;
-  uint8_t tmp[12];
   int n = 0;
   uint64_t digit = 0;
   do {
     digit = c % base + 1;
     if (rem && digit >= rem) digit++;
-    tmp[n++] = (uint8_t)digit;
+    n++;
     c /= base;
   } while (c);
-  for (int i = 0; i < n; i++) out->d[i] = tmp[n - 1 - i];
-  out->n = n;
-  out->pad0 = (rem && digit >= rem) ? 1 : 0;
-  return n;
+  *n_out = n;
+  *pad0_out = (rem && digit >= rem) ? 1 : 0;
+}
+
+// put(k0 + n - 1 - i, digit i) for the n digits of a count > 3, digit 0 the
+// least significant
+template <typename Put>
+__host__ __device__ static inline void gqf_put_counter_digits(uint64_t rem, uint64_t count, uint32_t k0, int n,
+                                                              Put put) {
+  const uint64_t base = rem ? 254 : 255;
+  uint64_t c = rem ? count - 3 : count - 4;
+  for (int i = 0; i < n; i++) {
+    uint64_t digit = c % base + 1;
+    if (rem && digit >= rem) digit++;
+    put(k0 + (uint32_t)(n - 1 - i), digit);
+    c /= base;
+  }
 }
 
 __host__ __device__ static inline uint32_t gqf_counter_len(uint64_t rem, uint64_t count) {
   if (count == 0) return 0;
   if (count <= 3) return (uint32_t)count;
-  gqf_digits g;
-  gqf_counter_digits(rem, count, &g);
-  return rem ? (uint32_t)(g.n + 2 + g.pad0) : (uint32_t)(g.n + 3);
+  int n, pad0;
+  gqf_counter_shape(rem, count, &n, &pad0);
+  return rem ? (uint32_t)(n + 2 + pad0) : (uint32_t)(n + 3);
 }
 
 // Decoder of one counter inside a run held as bytes run[0..len) whose last
@@ -322,12 +331,13 @@ __host__ __device__ static inline uint32_t gqf_emit_counter(uint64_t rem, uint64
     put(2, rem);
     return 3;
   }
-  gqf_digits g;
-  gqf_counter_digits(rem, count, &g);
+  int n, pad0;
+  gqf_counter_shape(rem, count, &n, &pad0);
   uint32_t k = 0;
   put(k++, rem);
-  if (rem && g.pad0) put(k++, 0);
-  for (int i = 0; i < g.n; i++) put(k++, g.d[i]);
+  if (rem && pad0) put(k++, 0);
+  gqf_put_counter_digits(rem, count, k, n, put);
+  k += (uint32_t)n;
   if (rem == 0) put(k++, 0);
   put(k++, rem);
   return k;
