@@ -166,7 +166,7 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
     n_dev = w.m_dev;
   }
   gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, w.region_list, w.list_count,
-                           s);
+                           ~0ULL, s);
   gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, n, w.region_start, w.region_list, w.list_count, mode,
                      w.wide_flags, w.wide_count, s);
   GQF_CHECK(hipGetLastError());
@@ -422,6 +422,85 @@ extern "C" void bulk_get_async(QF* qf, uint64_t nvals, uint64_t* vals, uint64_t*
   if (!f) return;
   gqf_launch_bulk_get(f->dev, vals, nvals, returns, (hipStream_t)stream);
   GQF_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// sequenced batches
+
+namespace {
+
+// request table of a sequenced call; `tagged`: req_tag carries seq << 1 | delete
+bool seq_requests(uint64_t n, uint32_t nreq, const uint32_t* req_end, const uint32_t* req_tag, bool tagged,
+                  gqf_seq_reqs& out) {
+  if (nreq == 0 || nreq > QF_SEQ_MAX_REQUESTS || !req_end || !req_tag || n > 0xFFFFFFFFULL) {
+    fprintf(stderr, "libgqf: sequenced call with %u requests (1..%d) or %llu keys\n", nreq, QF_SEQ_MAX_REQUESTS,
+            (unsigned long long)n);
+    return false;
+  }
+  uint32_t prev_end = 0;
+  int64_t prev_seq = -1;
+  for (uint32_t r = 0; r < nreq; r++) {
+    const uint32_t seq = tagged ? req_tag[r] >> 1 : req_tag[r];
+    if (req_end[r] < prev_end || (int64_t)seq <= prev_seq || seq > 0x7FFFu) {
+      fprintf(stderr, "libgqf: sequenced request %u: end %u after %u, sequence number %u after %lld\n", r,
+              req_end[r], prev_end, seq, (long long)prev_seq);
+      return false;
+    }
+    prev_end = req_end[r];
+    prev_seq = seq;
+  }
+  if ((uint64_t)prev_end != n) {
+    fprintf(stderr, "libgqf: sequenced requests end at %u, not at n = %llu\n", prev_end, (unsigned long long)n);
+    return false;
+  }
+  memset(&out, 0, sizeof(out));
+  out.nreq = nreq;
+  for (uint32_t r = 0; r < nreq; r++) {
+    out.end[r] = req_end[r];
+    out.tag[r] = req_tag[r];
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int qf_bulk_mutate_seq(QF* qf, uint64_t n, uint64_t* keys, uint32_t nreq, const uint32_t* req_end,
+                                  const uint32_t* req_tag, void* stream) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  if (f->host.key_bits + f->host.value_bits > 40) {
+    fprintf(stderr, "libgqf: sequenced batches take filters of at most 2^32 slots\n");
+    return -2;
+  }
+  gqf_seq_reqs reqs;
+  if (!seq_requests(n, nreq, req_end, req_tag, true, reqs)) return -3;
+  if (n == 0) return 0;
+  ensure_workspace(f, n);
+  Workspace& w = f->ws;
+  const hipStream_t s = (hipStream_t)stream;
+  gqf_launch_seq_tag(f->dev, keys, n, reqs, s);
+  // hash bits only: the tags above bit 48 ride along, and the passes are
+  // stable, so each hash's items stay in request order
+  gqf_launch_sort(keys, w.tmp, n, 0, 0, 0, (int)(f->host.key_bits + f->host.value_bits), f->sort_digit, w.hist,
+                  w.totals, s);
+  gqf_launch_region_bounds(f->dev, f->host.nregions, keys, n, nullptr, w.region_start, w.region_list, w.list_count,
+                           GQF_SEQ_HASH_MASK, s);
+  gqf_launch_rebuild(f->dev, f->host.nregions, keys, nullptr, n, w.region_start, w.region_list, w.list_count,
+                     GQF_REBUILD_SEQ, w.wide_flags, w.wide_count, s);
+  GQF_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int qf_bulk_get_seq_adjust(QF* qf, uint64_t n, const uint64_t* keys, uint32_t nreq, const uint32_t* req_end,
+                                      const uint32_t* req_seq, uint64_t* counts, uint64_t n_mut, const uint64_t* mut,
+                                      void* stream) {
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  gqf_seq_reqs reqs;
+  if (!seq_requests(n, nreq, req_end, req_seq, false, reqs)) return -3;
+  gqf_launch_seq_adjust(f->dev, keys, n, reqs, counts, mut, n_mut, (hipStream_t)stream);
+  GQF_CHECK(hipGetLastError());
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -782,7 +861,7 @@ int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF
                            sc.group_base, GQF_ENUM_OUT_MOVE, (int)src.value_bits, (int)dst.value_bits,
                            gqf_mask(src.key_bits), items, counts, nullptr, cap, s);
       gqf_launch_region_bounds(dst_dev, dst.nregions, items, n, nullptr, dst_ws.region_start, dst_ws.region_list,
-                               dst_ws.list_count, s);
+                               dst_ws.list_count, ~0ULL, s);
       gqf_launch_rebuild(dst_dev, dst.nregions, items, counts, n, dst_ws.region_start, dst_ws.region_list,
                          dst_ws.list_count, kModeInsert, dst_ws.wide_flags, dst_ws.wide_count, s);
     }

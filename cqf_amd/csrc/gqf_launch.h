@@ -24,7 +24,32 @@ void gqf_launch_rle(const uint64_t* keys, uint64_t n, uint32_t* tile_heads, uint
 // (2 lists of (nregions + 1) / 2 + 1 entries) with their counts
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
-                              uint32_t* list_count, hipStream_t s);
+                              uint32_t* list_count, uint64_t hmask, hipStream_t s);
+
+// Sequenced batches: a sorted item is its slot hash with the request tag
+// above GQF_SEQ_SHIFT -- the request's sequence number in bits 48..62 and
+// bit 63 set for a delete.  Even radix passes of any digit the sort takes
+// (6..10 bits) over at most 40 hash bits stay below bit 48.
+#define GQF_SEQ_SHIFT 48
+#define GQF_SEQ_HASH_MASK ((1ULL << GQF_SEQ_SHIFT) - 1)
+#define GQF_REBUILD_INSERT 0
+#define GQF_REBUILD_DELETE 1
+#define GQF_REBUILD_SEQ 2    // items tagged as above, applied in tag order per hash
+
+// request boundaries of a sequenced call, passed by value to its kernels:
+// request r owns keys [end[r - 1], end[r]) and has tag[r] = seq << 1 | delete
+struct gqf_seq_reqs {
+  uint32_t nreq;
+  uint32_t end[QF_SEQ_MAX_REQUESTS];
+  uint32_t tag[QF_SEQ_MAX_REQUESTS];
+};
+// keys[i] = slot hash of keys[i] | its request's tag
+void gqf_launch_seq_tag(const QF* qf_dev, uint64_t* keys, uint64_t n, const gqf_seq_reqs& reqs, hipStream_t s);
+// counts[i] (a count before the sequenced mutation `mut`, n_mut items sorted
+// by hash) += the effect of the mutation items of its hash whose request
+// precedes query i's request (reqs.tag holds the queries' sequence numbers)
+void gqf_launch_seq_adjust(const QF* qf_dev, const uint64_t* keys, uint64_t n, const gqf_seq_reqs& reqs,
+                           uint64_t* counts, const uint64_t* mut, uint64_t n_mut, hipStream_t s);
 
 // both even/odd phases (+ wide fallback passes) of a merge-rebuild over
 // the listed regions; n_items bounds the number of non-empty regions

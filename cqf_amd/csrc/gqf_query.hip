@@ -164,6 +164,95 @@ void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsig
 }
 
 // ---------------------------------------------------------------------------
+// Sequenced batches: one filter's requests in arrival order.  Mutation keys
+// become tagged slot hashes (gqf_seq_tag), are sorted by hash alone (stable,
+// so a hash's items stay in request order) and rebuilt in one pass; the
+// batch's queries are counted before the mutation and then corrected by
+// gqf_seq_adjust with the mutation items of the same hash that precede them.
+
+namespace {
+
+// the request boundaries into LDS, then request of key i = first r with end[r] > i
+struct SeqReqsLds {
+  uint32_t end[QF_SEQ_MAX_REQUESTS];
+  uint32_t tag[QF_SEQ_MAX_REQUESTS];
+};
+
+__device__ inline void seq_reqs_to_lds(const gqf_seq_reqs& reqs, SeqReqsLds& l) {
+  for (uint32_t r = threadIdx.x; r < reqs.nreq; r += blockDim.x) {
+    l.end[r] = reqs.end[r];
+    l.tag[r] = reqs.tag[r];
+  }
+  __syncthreads();
+}
+
+__device__ inline uint32_t seq_request_of(const SeqReqsLds& l, uint32_t nreq, uint64_t i) {
+  uint32_t lo = 0, hi = nreq;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)l.end[mid] > i) hi = mid; else lo = mid + 1;
+  }
+  return lo < nreq ? lo : nreq - 1;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256)
+gqf_seq_tag_kernel(const QF* __restrict__ qfp, uint64_t* __restrict__ keys, uint64_t n, gqf_seq_reqs reqs) {
+  __shared__ SeqReqsLds l;
+  seq_reqs_to_lds(reqs, l);
+  const QF qf = *qfp;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = l.tag[seq_request_of(l, reqs.nreq, i)];
+    keys[i] = gqf_key_hash(qf, keys[i]) | ((uint64_t)(t >> 1) << GQF_SEQ_SHIFT) | ((uint64_t)(t & 1u) << 63);
+  }
+}
+
+// Query i (of request with sequence number qs) replays, on its count before
+// the mutation, the items of its hash from requests before qs, in order:
+// +1 for an insert, -1 floored at 0 for a delete (the reference's qf_insert
+// and qf_remove on one hash, src/gqf.cu:1523-1933).
+extern "C" __global__ void __launch_bounds__(256)
+gqf_seq_adjust_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ keys, uint64_t n, gqf_seq_reqs reqs,
+                      uint64_t* __restrict__ counts, const uint64_t* __restrict__ mut, uint64_t n_mut) {
+  __shared__ SeqReqsLds l;
+  seq_reqs_to_lds(reqs, l);
+  const QF qf = *qfp;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t qs = l.tag[seq_request_of(l, reqs.nreq, i)];
+    const uint64_t h = gqf_key_hash(qf, keys[i]);
+    uint64_t lo = 0, hi = n_mut;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if ((mut[mid] & GQF_SEQ_HASH_MASK) < h) lo = mid + 1; else hi = mid;
+    }
+    if (lo == n_mut || (mut[lo] & GQF_SEQ_HASH_MASK) != h) continue;
+    uint64_t c = counts[i];
+    for (uint64_t j = lo; j < n_mut; j++) {
+      const uint64_t t = mut[j];
+      if ((t & GQF_SEQ_HASH_MASK) != h) break;
+      if (((t >> GQF_SEQ_SHIFT) & 0x7FFFu) >= qs) break;  // later requests of this hash follow in order
+      c = (t >> 63) ? (c ? c - 1 : 0) : c + 1;
+    }
+    counts[i] = c;
+  }
+}
+
+void gqf_launch_seq_tag(const QF* qf_dev, uint64_t* keys, uint64_t n, const gqf_seq_reqs& reqs, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(gqf_seq_tag_kernel, dim3(probe_grid(n)), dim3(256), 0, s, qf_dev, keys, n, reqs);
+}
+
+void gqf_launch_seq_adjust(const QF* qf_dev, const uint64_t* keys, uint64_t n, const gqf_seq_reqs& reqs,
+                           uint64_t* counts, const uint64_t* mut, uint64_t n_mut, hipStream_t s) {
+  if (n == 0 || n_mut == 0) return;
+  hipLaunchKernelGGL(gqf_seq_adjust_kernel, dim3(probe_grid(n)), dim3(256), 0, s, qf_dev, keys, n, reqs, counts, mut,
+                     n_mut);
+}
+
+// ---------------------------------------------------------------------------
 // synthetic key streams for benchmarks and tests: out[i] = splitmix64 output
 // number (skip + i) of the stream seeded with `seed` (Steele, Lea & Flood
 // 2014; the state advances by the golden gamma, then two xor-shift-multiply

@@ -39,7 +39,7 @@ constexpr int kWinWide = 256;         // fallback window (208 push blocks)
 constexpr int kItemCap = 320;         // items cached in LDS per region (fast path)
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
-enum RebuildMode : int { kInsert = 0, kDelete = 1 };
+enum RebuildMode : int { kInsert = 0, kDelete = 1, kSeq = 2 };
 enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 
 template <int W>
@@ -85,22 +85,36 @@ __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
 
 
 // The region's items: cached relative hashes in LDS, or (when the region
-// holds more than kItemCap) read from global memory.  Counts (reduce mode)
-// are cached in a separate LDS array that only the counting kernels
-// declare, so the plain insert and delete kernels keep their LDS small.
+// holds more than kItemCap) read from global memory.  Each item changes its
+// hash's count by a delta: +1 (insert), -1 (delete), +count (reduce mode,
+// counts array), or +1 / -1 by the item's tag bit 63 (sequenced mode, whose
+// items carry a request tag above bit GQF_SEQ_SHIFT that hmask strips).
+// Deltas other than +-1 are cached in a separate LDS array that only the
+// counting kernels declare, so the plain insert and delete kernels keep
+// their LDS small.
+enum ItemKind : int { kAdd = 0, kSub = 1, kCounts = 2, kTagged = 3 };
+
 struct Items {
   const uint64_t* __restrict__ g;
   const uint64_t* __restrict__ gc;
-  uint32_t* lcnt;  // LDS cache of the counts (cached and gc != nullptr)
-  uint64_t s;     // first item index of the region
-  uint64_t base;  // first hash of the region
+  uint32_t* lcnt;  // LDS cache of the deltas (cached, kinds kCounts and kTagged)
+  uint64_t s;      // first item index of the region
+  uint64_t base;   // first hash of the region
+  uint64_t hmask;  // hash bits of an item
+  int kind;
   bool cached;
   __device__ inline uint32_t hash(const uint32_t* lds, uint32_t i) const {
-    return cached ? lds[i] : (uint32_t)(g[s + i] - base);
+    return cached ? lds[i] : (uint32_t)((g[s + i] & hmask) - base);
   }
-  __device__ inline uint64_t count(uint32_t i) const {
-    if (!gc) return 1;
-    return cached ? (uint64_t)lcnt[i] : gc[s + i];
+  __device__ inline int64_t delta(uint32_t i) const {
+    if (kind == kAdd) return 1;
+    if (kind == kSub) return -1;
+    if (kind == kCounts) return cached ? (int64_t)lcnt[i] : (int64_t)gc[s + i];
+    return cached ? (int64_t)(int32_t)lcnt[i] : ((g[s + i] >> 63) ? -1 : 1);
+  }
+  // the value cached in lcnt for item i (global index s + i)
+  __device__ inline uint32_t cache_delta(uint64_t i) const {
+    return kind == kCounts ? (uint32_t)gc[i] : ((g[i] >> 63) ? 0xFFFFFFFFu : 1u);
   }
 };
 
@@ -108,14 +122,19 @@ struct Items {
 // Two-pointer merge of the old counters of one quotient (ascending
 // remainders, old run at st_o .. st_o + old_len - 1 of sin) with its items
 // [it0, it1) (ascending hashes).  Returns the new run length; with WRITE the
-// counters are written from st_n into raw.  Insert adds the counts,
-// delete subtracts them (floored at 0; deleting an absent key is a no-op),
-// as sequences of the reference's qf_insert / qf_remove would
-// (src/gqf.cu:1523-1933).
+// counters are written from st_n into raw.
+//
+// The items of one hash are applied in order, as a sequence of the
+// reference's qf_insert (count + 1, src/gqf.cu:1523-1752) and qf_remove
+// (count - 1, floored at 0: removing an absent key is a no-op,
+// src/gqf.cu:1755-1933).  Each step maps the count x to max(m, x + s), and
+// such maps compose: +d gives (m + d, s + d), -d gives (max(0, m - d), s - d),
+// from the identity (0, 0).  m >= s and m >= 0 throughout, so the new count
+// is old + s when old >= m - s, else m.
 
 template <int W, bool WRITE>
 __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
-                                                   int32_t st_o, uint32_t old_len, int32_t st_n, int mode) {
+                                                   int32_t st_o, uint32_t old_len, int32_t st_n) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
   uint32_t len = 0;
@@ -127,7 +146,8 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
   uint64_t o_rem = 0, o_cnt = 0;
   bool have_old = old_len > 0;
   if (have_old) o_end = gqf_decode_counter(old_slot, old_len, 0, &o_rem, &o_cnt);
-  uint64_t n_rem = 0, n_cnt = 0;
+  uint64_t n_rem = 0;
+  int64_t n_m = 0, n_s = 0;
   bool have_new = false;
   uint32_t it = it0;
   auto next_new = [&]() {
@@ -135,9 +155,12 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
     if (!have_new) return;
     const uint32_t h0 = src.hash(sm.items, it);
     n_rem = h0 & 0xFF;
-    n_cnt = 0;
+    n_m = 0;
+    n_s = 0;
     while (it < it1 && src.hash(sm.items, it) == h0) {
-      n_cnt += src.count(it);
+      const int64_t d = src.delta(it);
+      n_s += d;
+      n_m = (d > 0 || n_m + d > 0) ? n_m + d : 0;
       it++;
     }
   };
@@ -155,10 +178,10 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
       len += clen;
       next_old();
     } else if (!have_old || n_rem < o_rem) {
-      if (mode == kInsert) len += gqf_emit_counter(n_rem, n_cnt, put);
+      len += gqf_emit_counter(n_rem, (uint64_t)n_m, put);  // from count 0: max(m, s) = m
       next_new();
     } else {
-      const uint64_t c = mode == kInsert ? o_cnt + n_cnt : (o_cnt > n_cnt ? o_cnt - n_cnt : 0);
+      const uint64_t c = o_cnt >= (uint64_t)(n_m - n_s) ? o_cnt + (uint64_t)n_s : (uint64_t)n_m;
       len += gqf_emit_counter(n_rem, c, put);
       next_old();
       next_new();
@@ -174,7 +197,7 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
 
 template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const Items& src, uint32_t it, uint32_t ie,
-                           int mode, uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
+                           uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
   const int32_t blk_base = 64 * j;
@@ -230,7 +253,7 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       const uint32_t it0 = it;
       while (it < ie && (int32_t)(src.hash(sm.items, it) >> 8) == qa) it++;
       cur = it < ie ? src.hash(sm.items, it) : 0;
-      len = merge_quotient<W, WRITE>(sm, src, it0, it, st_o, old_len, st_n, mode);
+      len = merge_quotient<W, WRITE>(sm, src, it0, it, st_o, old_len, st_n);
     }
 
     if (len) {
@@ -384,7 +407,9 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   const int64_t win_base = (int64_t)(B0 * 64);
   const bool table_end_in_window = B0 + (uint64_t)nwin >= qf.nblocks;
   const uint32_t m = (uint32_t)(e - s);
-  const Items src{items, counts, lcnt, s, tag + (B0 << 14), m <= (uint32_t)kItemCap};
+  const uint64_t hmask = mode == kSeq ? GQF_SEQ_HASH_MASK : ~0ULL;
+  const int kind = mode == kSeq ? kTagged : (counts ? kCounts : (mode == kDelete ? kSub : kAdd));
+  const Items src{items, counts, lcnt, s, tag + (B0 << 14), hmask, kind, m <= (uint32_t)kItemCap};
 
   // ---- 1. stage the window and the items ----
   // lane j reads block j's line: header into registers, slots into LDS row j
@@ -408,8 +433,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     }
     if (src.cached) {
       for (uint32_t i = lane; i < m; i += 64) {
-        sm.items[i] = (uint32_t)(items[s + i] - src.base);
-        if (counts) lcnt[i] = (uint32_t)counts[s + i];
+        sm.items[i] = (uint32_t)((items[s + i] & hmask) - src.base);
+        if (kind == kCounts || kind == kTagged) lcnt[i] = src.cache_delta(s + i);
       }
     }
     sm.blk_cnt[lane] = 0;
@@ -482,7 +507,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       uint32_t nl = 0;
       if (locate_old_run(sm, nwin, q, st_o, old_len)) {
         const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0, mode);
+        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0);
       } else {
         atomicMin(&sm.first_invalid, q >> 6);
         st_o = q;
@@ -512,7 +537,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
           ie = it + sm.blk_cnt[j];
         }
         const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], a, b, occ_new[k])
-                             : walk_block<W, false>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occ_new[k]);
+                             : walk_block<W, false>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occ_new[k]);
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
       int32_t ia = a, ib = b;
@@ -607,7 +632,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       int32_t a, b;
       uint64_t occn;
       if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], a, b, occn);
-      else walk_block<W, true>(sm, j, nwin, src, it, ie, mode, occ_w[k], a, b, occn);
+      else walk_block<W, true>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occn);
     }
   }
   __syncthreads();
@@ -617,7 +642,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     for (uint32_t t = lane; t < nq; t += 64) {
       if (sm.tnl[t] == 0) continue;
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t], mode);
+      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t]);
     }
   }
   __syncthreads();
@@ -742,15 +767,15 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   if (lane == 0) *wide_count = 0;
 }
 
-// region_start[r] = first index i with items[i] >= r * (region_blocks * 64 * 256), for
-// r in [0, nregions]; items at or above `range` fall past the last region.
+// region_start[r] = first index i with (items[i] & hmask) >= r * (region_blocks * 64 * 256),
+// for r in [0, nregions]; items at or above `range` fall past the last region.
 // Also appends each non-empty region r < nregions to region_list (at
 // parity r & 1, list stride `list_stride`), counting in list_count[parity]
 // (zeroed by gqf_region_lists_reset just before).
 extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                                              uint64_t n_host, const uint64_t* __restrict__ n_dev,
                                              uint32_t* __restrict__ region_start, uint32_t* __restrict__ region_list,
-                                             uint32_t* __restrict__ list_count, uint64_t list_stride) {
+                                             uint32_t* __restrict__ list_count, uint64_t list_stride, uint64_t hmask) {
   const QF qf = *qfp;
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = r <= qf.nregions;
@@ -759,7 +784,7 @@ extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const u
     uint64_t lo = 0, hi = n;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) >> 1;
-      if (items[mid] < bound) lo = mid + 1; else hi = mid;
+      if ((items[mid] & hmask) < bound) lo = mid + 1; else hi = mid;
     }
     return lo;
   };
@@ -938,12 +963,12 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
 
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
-                              uint32_t* list_count, hipStream_t s) {
+                              uint32_t* list_count, uint64_t hmask, hipStream_t s) {
   const uint32_t g = (uint32_t)((nregions + 1 + 255) / 256);
   gqf_prof_begin("region_bounds", s);
   hipLaunchKernelGGL(gqf_region_lists_reset, dim3(1), dim3(64), 0, s, list_count);
   hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(256), 0, s, qf_dev, items, n, n_dev, region_start,
-                     region_list, list_count, (uint64_t)((nregions + 1) / 2 + 1));
+                     region_list, list_count, (uint64_t)((nregions + 1) / 2 + 1), hmask);
   gqf_prof_end("region_bounds", s);
 }
 
@@ -958,7 +983,7 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
   const uint64_t stride = per_parity + 1;
   for (int parity = 0; parity < 2; parity++) {
     gqf_prof_begin("rebuild_phase", s);
-    if (counts)
+    if (counts || mode == kSeq)
       hipLaunchKernelGGL(gqf_rebuild_phase_counts, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start,
                          region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
     else

@@ -41,7 +41,10 @@ EXPORTS = [
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
     "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
+    "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust",
 ]
+
+QF_SEQ_MAX_REQUESTS = 256
 
 QF_ENUM_HASH = 0
 QF_ENUM_KEY = 1
@@ -138,6 +141,11 @@ def load_library(path=LIB_PATH):
     L.qf_merge_device.argtypes = [_vp, _vp, _vp]
     L.qf_merge_device.restype = ctypes.c_int64
     L.qf_multi_merge_device.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _vp]
+    _u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.qf_bulk_mutate_seq.argtypes = [_vp, _u64, _vp, ctypes.c_uint32, _u32p, _u32p, _vp]
+    L.qf_bulk_mutate_seq.restype = ctypes.c_int
+    L.qf_bulk_get_seq_adjust.argtypes = [_vp, _u64, _vp, ctypes.c_uint32, _u32p, _u32p, _vp, _u64, _vp, _vp]
+    L.qf_bulk_get_seq_adjust.restype = ctypes.c_int
     L.qf_multi_merge_device.restype = ctypes.c_int64
     _lib = L
     return L
@@ -229,6 +237,32 @@ class GQF:
             returns = torch.empty_like(vals)
         self._lib.bulk_get_async(self.handle, vals.numel(), _ptr(vals), _ptr(returns), _stream_ptr(stream))
         return returns
+
+    # -- sequenced batches: requests of one filter in arrival order
+    @staticmethod
+    def _u32_array(vals):
+        return (ctypes.c_uint32 * len(vals))(*vals)
+
+    def mutate_seq(self, keys, ends, tags, stream=None):
+        """Inserts and deletes of several requests as one rebuild: request r
+        owns keys[ends[r-1]:ends[r]] and has tags[r] = seq << 1 | is_delete
+        (seq ascending); the result is the requests applied one at a time.
+        keys is left holding the tagged hashes sorted by hash."""
+        rc = self._lib.qf_bulk_mutate_seq(self.handle, keys.numel(), _ptr(keys), len(ends), self._u32_array(ends),
+                                          self._u32_array(tags), _stream_ptr(stream))
+        if rc != 0:
+            raise ValueError(f"qf_bulk_mutate_seq rc={rc}")
+
+    def get_seq_adjust(self, keys, ends, seqs, counts, mut_sorted, stream=None):
+        """counts (bulk_get of keys before mutate_seq) -> the counts after the
+        mutation requests whose sequence numbers precede each query's
+        request (seqs[r] of query request r owning keys[ends[r-1]:ends[r]])."""
+        rc = self._lib.qf_bulk_get_seq_adjust(self.handle, keys.numel(), _ptr(keys), len(ends), self._u32_array(ends),
+                                              self._u32_array(seqs), _ptr(counts), mut_sorted.numel(),
+                                              _ptr(mut_sorted), _stream_ptr(stream))
+        if rc != 0:
+            raise ValueError(f"qf_bulk_get_seq_adjust rc={rc}")
+        return counts
 
     def bulk_get_misses(self, vals):
         return int(self._lib.bulk_get_misses_wrapper(self.handle, _ptr(vals), vals.numel()))

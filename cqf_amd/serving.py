@@ -6,18 +6,23 @@ loop keeps every admitted filter resident in HBM and, each round:
 
   * admits arrived requests into a FIFO per filter;
   * for every filter whose previous batch has left the device (at most
-    `max_inflight` batches in flight per filter), takes the longest run of
-    requests of one kind at the head of its FIFO (up to `max_batch_keys`
-    keys) and launches them as ONE bulk call on the filter's stream --
-    inserts and deletes concatenated into one bulk_insert / bulk_delete,
-    queries into one bulk_get whose result is split back per request;
+    `max_inflight` batches in flight per filter), takes the requests at the
+    head of its FIFO -- of every kind, in arrival order, up to
+    `max_batch_keys` keys and 256 requests of each side -- and launches
+    them as one sequenced batch on the filter's stream: ONE bulk_get of all
+    the batch's query keys, taken before the batch's mutations; ONE
+    sequenced mutation (qf_bulk_mutate_seq) applying all its inserts and
+    deletes in request order with one sort and one rebuild; and, when a
+    query follows a mutation in the batch, ONE correction
+    (qf_bulk_get_seq_adjust) that replays on each query's count the
+    mutations of its hash that arrived before it;
   * filters are spread over `n_streams` HIP streams, so batches of
     different filters overlap on the device while requests keep arriving.
 
 Per-filter order: a filter's batches are issued in FIFO order on a single
-stream and a run never crosses a request of another kind, so every query
-sees exactly the mutations that arrived before it -- the same result as the
-requests applied one by one in arrival order ("alone").
+stream, and within a batch every query's count is the count after exactly
+the mutations that arrived before it -- the same result as the requests
+applied one by one in arrival order ("alone").
 
 Admission: a filter is created on its first request if its table and
 workspace fit the HBM budget; otherwise the least recently used idle filter
@@ -128,28 +133,57 @@ class Server:
                 done += len(reqs)
         return done
 
+    def _take(self, f):
+        """The FIFO prefix of one sequenced batch: (gets, mutations), each a
+        list of (sequence number, request) in arrival order."""
+        gets, muts = [], []
+        n_get = n_mut = 0
+        seq = 0
+        while f.fifo:
+            r = f.fifo[0]
+            k = r.keys.numel()
+            if r.op == "get":
+                if len(gets) == G.QF_SEQ_MAX_REQUESTS or (gets and n_get + k > self.max_batch):
+                    break
+                gets.append((seq, r))
+                n_get += k
+            else:
+                if len(muts) == G.QF_SEQ_MAX_REQUESTS or (muts and n_mut + k > self.max_batch):
+                    break
+                muts.append((seq, r))
+                n_mut += k
+            f.fifo.popleft()
+            seq += 1
+        return gets, muts
+
+    @staticmethod
+    def _concat(side):
+        keys = [r.keys for _, r in side]
+        batch = torch.cat(keys) if len(keys) > 1 else keys[0].clone()
+        ends = np.cumsum([k.numel() for k in keys]).tolist()
+        return batch, ends
+
     def _launch(self, f, now):
-        head = f.fifo[0].op
-        reqs, n = [], 0
-        while f.fifo and f.fifo[0].op == head and (not reqs or n + f.fifo[0].keys.numel() <= self.max_batch):
-            r = f.fifo.popleft()
-            reqs.append(r)
-            n += r.keys.numel()
+        gets, muts = self._take(f)
+        reqs = [r for _, r in sorted(gets + muts, key=lambda x: x[0])]
         s = f.stream
         with torch.cuda.stream(s):
-            batch = torch.cat([r.keys for r in reqs]) if len(reqs) > 1 else reqs[0].keys.clone()
-            keep = [batch]
-            if head == "get":
-                out = f.q.bulk_get(batch, stream=s)
-                keep.append(out)
+            keep = []
+            if gets:
+                gk, g_ends = self._concat(gets)
+                out = f.q.bulk_get(gk, stream=s)  # the counts before this batch's mutations
+                keep += [gk, out]
+            if muts:
+                mk, m_ends = self._concat(muts)
+                f.q.mutate_seq(mk, m_ends, [(q << 1) | (r.op == "delete") for q, r in muts], s)
+                keep.append(mk)
+                if gets and gets[-1][0] > muts[0][0]:  # a query follows a mutation
+                    f.q.get_seq_adjust(gk, g_ends, [q for q, _ in gets], out, mk, s)
+            if gets:
                 o = 0
-                for r in reqs:
+                for _, r in gets:
                     r.result = out[o:o + r.keys.numel()]
                     o += r.keys.numel()
-            elif head == "insert":
-                f.q.bulk_insert(batch, 0, s)
-            else:
-                f.q.bulk_delete(batch, 0, s)
             ev = torch.cuda.Event()
             ev.record(s)
         for r in reqs:

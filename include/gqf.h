@@ -129,6 +129,41 @@ void bulk_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, vo
 void bulk_insert_reduce_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_delete_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_get_async(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns, void *stream);
+
+/* ---- sequenced batches (no reference counterpart) -------------------
+ * One filter's requests, in arrival order, as a few launches instead of
+ * one bulk call each, with the results of the calls one by one.  For
+ * serving loops that coalesce requests of all kinds.
+ *
+ * A call covers nreq <= QF_SEQ_MAX_REQUESTS requests of one device key
+ * array: request r owns keys [req_end[r - 1], req_end[r]) (req_end[-1] = 0,
+ * req_end[nreq - 1] = n; host array) and has sequence number seq_r, which
+ * must ascend with r and stay below 2^15.
+ *
+ * qf_bulk_mutate_seq applies inserts and deletes (req_tag[r] = seq_r << 1,
+ * | 1 for a delete; host array) as bulk_insert / bulk_delete called one
+ * request at a time: on each hash, inserts add 1 and deletes remove 1 down
+ * to 0, in request order.  One sort and one rebuild for all requests.
+ * keys is left holding the tagged hashes sorted by hash: slot hash | seq_r
+ * << 48 | delete << 63 (the input of qf_bulk_get_seq_adjust).  Filters of
+ * at most 2^32 slots (key bits + value bits <= 40).
+ *
+ * qf_bulk_get_seq_adjust completes queries of the same batch: counts
+ * holds bulk_get of the query keys taken BEFORE the mutation (stream
+ * order), req_seq[r] the sequence number of query request r; each count
+ * becomes the count after exactly the mutation requests whose sequence
+ * numbers are below its request's.  mut / n_mut: the sorted keys the
+ * mutation call left.
+ *
+ * Both return 0, or a negative value for bad arguments (nothing launched).
+ * Stream-ordered; calls on one filter must use one stream. */
+#define QF_SEQ_MAX_REQUESTS 256
+int qf_bulk_mutate_seq(QF *qf, uint64_t n, uint64_t *keys, uint32_t nreq, const uint32_t *req_end,
+                       const uint32_t *req_tag, void *stream);
+int qf_bulk_get_seq_adjust(QF *qf, uint64_t n, const uint64_t *keys, uint32_t nreq, const uint32_t *req_end,
+                           const uint32_t *req_seq, uint64_t *counts, uint64_t n_mut, const uint64_t *mut,
+                           void *stream);
+
 /* ---- filter groups and tuning (no reference counterpart) ---------- */
 
 /* A group of existing filters served by shared launches.  keys / vals are
