@@ -167,8 +167,8 @@ def zipf_main(args):
         step(recs)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "region_bounds", "rebuild_phase", "rebuild_wide",
-                                              "bulk_get")}
+    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "sort_buckets", "region_bounds", "rebuild_phase",
+                                              "rebuild_wide", "bulk_get")}
     gqf.profile_enable(False)
     ins = [e[0].elapsed_time(e[1]) for e in recs]
     qry = [e[1].elapsed_time(e[2]) for e in recs]
@@ -642,7 +642,7 @@ def main():
         del_ms += evs[nb + 2].elapsed_time(evs[nb + 2 + nd])
 
     # kernel-level times from the library's own events
-    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "region_bounds", "rebuild_phase",
+    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "sort_buckets", "region_bounds", "rebuild_phase",
                                               "rebuild_wide", "bulk_get")}
     gqf.profile_enable(False)
 

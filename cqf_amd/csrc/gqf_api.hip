@@ -143,8 +143,11 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
   const int vb = (int)f->host.value_bits;
   const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
-  gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
-                  w.hist, w.totals, s);
+  if (hash_it && gqf_sort_two_level_fits(n, bits))
+    gqf_launch_sort_two_level(keys, w.tmp, n, gqf_mask(f->host.key_bits), vb, bits, w.hist, w.totals, s);
+  else
+    gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
+                    w.hist, w.totals, s);
   // A sparse insert batch -- at most one key per point-lock region and one
   // per two rebuild regions on average -- would make the rebuild rewrite a
   // whole window for about one key each; the point path inserts each key

@@ -13,6 +13,13 @@ uint64_t gqf_sort_hist_words(uint64_t n);
 // digit_bits: 6..10 fixes the radix digit (passes rounded up to even), 0 picks 4 passes
 void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int vshift, int bits,
                      int digit_bits, uint32_t* hist, uint32_t* totals, hipStream_t s);
+// the same result for keys to be hashed (hash_it 1), for batches where
+// gqf_sort_two_level_fits: one stable pass on the top hash bits into
+// buckets of about 12K keys, then an LDS sort per bucket; not stable on
+// equal hashes (plain keys only: equal hashes are indistinguishable)
+bool gqf_sort_two_level_fits(uint64_t n, int bits);
+void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
+                               uint32_t* hist, uint32_t* totals, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
 // tile_heads: ceil(n/4096) words; upos: n words.

@@ -269,6 +269,153 @@ gqf_sort_downsweep_small(const uint64_t* __restrict__ in, uint64_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Two-level sort of hashed keys (batches of 2^20..2^24): one stable pass of
+// the machinery above on the top `msd` hash bits scatters the keys into
+// 2^msd buckets; then one workgroup per bucket sorts the bucket's remaining
+// low bits (at most 32) in LDS -- up to kBucketCap keys as 32-bit values in
+// two 72 KB buffers of gfx950's 160 KB -- with stable 8-bit LSD passes whose
+// ranks come from wave ballots over peers of equal digit, as in the
+// downsweep.  The bucket's top bits are its index, so the low bits order it.
+// A bucket larger than kBucketCap (heavily repeated keys) runs the same
+// passes over its 64-bit keys in global memory.
+
+namespace {
+constexpr int kBThreads = 512;
+constexpr int kBWaves = kBThreads / 64;
+constexpr int kBItems = 8;                           // keys per lane per round
+constexpr uint32_t kBRound = kBThreads * kBItems;    // 4096
+constexpr uint32_t kBucketCap = 18432;
+constexpr int kBMaxPasses = 5;
+
+struct BucketSmem {
+  uint32_t buf[2][kBucketCap];
+  uint32_t wcnt[kBWaves][256];        // per wave digit counts, then positions
+  uint32_t tot[kBMaxPasses][256];     // digit totals of every pass
+  uint32_t run[256];                  // next free position of each digit
+};
+
+// Stable scatter of src[0, m) into dst by digit (v >> sh) & 255, given the
+// digit totals; every thread of the workgroup calls it.
+template <typename V>
+__device__ __forceinline__ void bucket_pass(const V* src, V* dst, uint32_t m, int sh, const uint32_t* tot,
+                                            BucketSmem& sm) {
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (w == 0) {  // exclusive scan of the totals, 4 digits per lane
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v[k] = tot[4 * lane + k];
+      sum += v[k];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t x = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += x;
+    }
+    uint32_t r = incl - sum;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      sm.run[4 * lane + k] = r;
+      r += v[k];
+    }
+  }
+  __syncthreads();
+  const uint64_t lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  for (uint32_t base = 0; base < m; base += kBRound) {
+    for (int d = lane; d < 256; d += 64) sm.wcnt[w][d] = 0;
+    V keys[kBItems];
+    uint32_t rank[kBItems];
+#pragma unroll
+    for (int k = 0; k < kBItems; k++) {
+      const uint32_t i = base + (uint32_t)(w * 64 * kBItems + k * 64 + lane);
+      const bool valid = i < m;
+      const V key = valid ? src[i] : (V)0;
+      const uint32_t d = (uint32_t)(key >> sh) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const uint64_t mb = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? mb : ~mb;
+      }
+      const uint32_t before = sm.wcnt[w][d];
+      rank[k] = before + (uint32_t)__popcll(peers & lt);
+      keys[k] = key;
+      if (valid && (peers & lt) == 0) sm.wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    if (t < 256) {  // wave counts -> positions, in wave order
+      uint32_t r = sm.run[t];
+#pragma unroll
+      for (int k = 0; k < kBWaves; k++) {
+        const uint32_t x = sm.wcnt[k][t];
+        sm.wcnt[k][t] = r;
+        r += x;
+      }
+      sm.run[t] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBItems; k++) {
+      const uint32_t i = base + (uint32_t)(w * 64 * kBItems + k * 64 + lane);
+      if (i < m) dst[sm.wcnt[w][(uint32_t)(keys[k] >> sh) & 255u] + rank[k]] = keys[k];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(kBThreads)
+gqf_sort_buckets(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
+                 uint32_t nbuckets) {
+  __shared__ BucketSmem sm;
+  const uint32_t b = blockIdx.x;
+  const uint64_t s = starts[b];
+  const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
+  if (e <= s) return;
+  const uint32_t m = (uint32_t)(e - s);
+  const int t = threadIdx.x;
+  const int passes = (low_bits + 7) / 8;
+  for (int k = t; k < kBMaxPasses * 256; k += kBThreads) (&sm.tot[0][0])[k] = 0;
+  __syncthreads();
+  if (m <= kBucketCap) {
+    const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
+    for (uint32_t i = t; i < m; i += kBThreads) {
+      const uint32_t v = (uint32_t)(in[s + i] - hi);
+      sm.buf[0][i] = v;
+      for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    int src = 0;
+    for (int p = 0; p < passes; p++) {
+      bucket_pass<uint32_t>(sm.buf[src], sm.buf[src ^ 1], m, 8 * p, sm.tot[p], sm);
+      __syncthreads();
+      src ^= 1;
+    }
+    for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | sm.buf[src][i];
+    return;
+  }
+  // too large for LDS: the same passes between the bucket's ranges of in and
+  // out, plus a copy pass on the (shared) top bits when the count is even, so
+  // the result lands in out; other waves' stores are published between
+  // passes with agent-scope fences around the barrier
+  const int gpasses = passes | 1;
+  uint64_t* a = in + s;
+  uint64_t* o = out + s;
+  for (uint32_t i = t; i < m; i += kBThreads) {
+    const uint64_t v = a[i];
+    for (int p = 0; p < gpasses; p++) atomicAdd(&sm.tot[p][(uint32_t)(v >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < gpasses; p++) {
+    bucket_pass<uint64_t>(p & 1 ? o : a, p & 1 ? a : o, m, 8 * p, sm.tot[p], sm);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+}
+
+// ---------------------------------------------------------------------------
 // run-length reduce of sorted keys: unique keys + their multiplicities
 
 extern "C" __global__ void __launch_bounds__(kThreads)
@@ -411,6 +558,32 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
     in = out;
     out = t;
   }
+}
+
+bool gqf_sort_two_level_fits(uint64_t n, int bits) {
+  return n >= (1ULL << 20) && n <= (1ULL << 24) && bits >= 18 && bits <= 42;
+}
+
+void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
+                               uint32_t* hist, uint32_t* totals, hipStream_t s) {
+  // 2^msd buckets of about 12K keys (at most 1024), low bits <= 32
+  int msd = 4;
+  while (msd < 10 && (n >> msd) > 12288) msd++;
+  if (bits - msd > 32) msd = bits - 32;
+  const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
+  const int radix = 1 << msd;
+  gqf_prof_begin("sort_pass", s);
+  hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)keys, n, bits - msd, msd,
+                     1, mask, vshift, hist, ntiles);
+  hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
+  hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
+  hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)keys, tmp, n, bits - msd,
+                     msd, 1, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+  gqf_prof_end("sort_pass", s);
+  gqf_prof_begin("sort_buckets", s);
+  hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
+                     (const uint32_t*)totals, (uint32_t)radix);
+  gqf_prof_end("sort_buckets", s);
 }
 
 void gqf_launch_rle(const uint64_t* keys, uint64_t n, uint32_t* tile_heads, uint64_t* ukeys, uint64_t* upos,

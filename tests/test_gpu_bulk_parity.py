@@ -249,3 +249,57 @@ def test_sparse_batches_after_dense_fill(gpu, oracle, nbits):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("nbits,sizes", [(22, [1 << 21]), (23, [int(0.9 * (1 << 23))]),
+                                         (22, [1 << 20, 1 << 21])])
+def test_large_batches_two_level_sort(gpu, oracle, nbits, sizes):
+    """Batches of 2^20..2^24 keys take the two-level sort (top-bit buckets,
+    then one LDS sort per bucket): keys must come back as the ascending
+    hashes and the image must equal the oracle's."""
+    from cqf_amd.gqf import GQF, to_device_u64, to_numpy_u64
+    _say(f"case two-level sort q={nbits} batches={sizes}")
+    g, o = GQF(nbits), oracle.OracleQF(nbits)
+    try:
+        off = 0
+        for n in sizes:
+            keys = oracle.splitmix64(41, n, skip=off)
+            off += n
+            dk = to_device_u64(keys)
+            g.bulk_insert(dk)
+            hk = keys.copy()
+            o.bulk_insert(hk)
+            assert np.array_equal(to_numpy_u64(dk), hk)
+        _assert_images_equal(g.export_image(), o.image(), "image after two-level sorted batches")
+        assert g.status() == 0
+        q = oracle.splitmix64(41, 1 << 16)
+        assert np.array_equal(to_numpy_u64(g.bulk_get(to_device_u64(q))), o.bulk_get(q))
+    finally:
+        g.close()
+        o.close()
+
+
+def test_two_level_sort_repeated_keys(gpu, oracle):
+    """2^20 copies of 16 keys: each key's bucket holds far more keys than the
+    LDS sort takes, so those buckets are sorted in global memory."""
+    from cqf_amd.gqf import GQF, to_device_u64, to_numpy_u64
+    nbits = 20
+    _say("case two-level sort, 16 keys x 2^16 copies")
+    univ = oracle.splitmix64(43, 16)
+    keys = np.repeat(univ, 1 << 16)
+    keys = keys[oracle.splitmix64(44, keys.size) % np.uint64(keys.size)]  # shuffled with repeats
+    g, o = GQF(nbits), oracle.OracleQF(nbits)
+    try:
+        dk = to_device_u64(keys)
+        g.bulk_insert(dk)
+        hk = keys.copy()
+        o.bulk_insert(hk)
+        assert np.array_equal(to_numpy_u64(dk), hk)
+        _assert_images_equal(g.export_image(), o.image(), "image after repeated keys")
+        assert np.array_equal(to_numpy_u64(g.bulk_get(to_device_u64(univ))), o.bulk_get(univ))
+        g.bulk_delete(to_device_u64(keys))
+        gpu.cuda.synchronize()
+        assert not g.export_image().any()
+    finally:
+        g.close()
+        o.close()
