@@ -273,18 +273,18 @@ gqf_sort_downsweep_small(const uint64_t* __restrict__ in, uint64_t* __restrict__
 // the machinery above on the top `msd` hash bits scatters the keys into
 // 2^msd buckets; then one workgroup per bucket sorts the bucket's remaining
 // low bits (at most 32) in LDS -- up to kBucketCap keys as 32-bit values in
-// two 72 KB buffers of gfx950's 160 KB -- with stable 8-bit LSD passes whose
+// two 68 KB buffers of gfx950's 160 KB -- with stable 8-bit LSD passes whose
 // ranks come from wave ballots over peers of equal digit, as in the
 // downsweep.  The bucket's top bits are its index, so the low bits order it.
 // A bucket larger than kBucketCap (heavily repeated keys) runs the same
 // passes over its 64-bit keys in global memory.
 
 namespace {
-constexpr int kBThreads = 512;
+constexpr int kBThreads = 1024;
 constexpr int kBWaves = kBThreads / 64;
 constexpr int kBItems = 8;                           // keys per lane per round
-constexpr uint32_t kBRound = kBThreads * kBItems;    // 4096
-constexpr uint32_t kBucketCap = 18432;
+constexpr uint32_t kBRound = kBThreads * kBItems;    // 8192
+constexpr uint32_t kBucketCap = 17408;               // 2^24 keys in 1024 buckets: mean 16384, sd 128
 constexpr int kBMaxPasses = 5;
 
 struct BucketSmem {
@@ -380,10 +380,21 @@ gqf_sort_buckets(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const ui
   __syncthreads();
   if (m <= kBucketCap) {
     const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
-    for (uint32_t i = t; i < m; i += kBThreads) {
-      const uint32_t v = (uint32_t)(in[s + i] - hi);
-      sm.buf[0][i] = v;
-      for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v >> (8 * p)) & 255u], 1u);
+    for (uint32_t base = 0; base < m; base += kBThreads * kBItems) {
+      uint32_t v[kBItems];  // all loads in flight before any is used
+#pragma unroll
+      for (int k = 0; k < kBItems; k++) {
+        const uint32_t i = base + (uint32_t)(k * kBThreads + t);
+        v[k] = i < m ? (uint32_t)(in[s + i] - hi) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kBItems; k++) {
+        const uint32_t i = base + (uint32_t)(k * kBThreads + t);
+        if (i < m) {
+          sm.buf[0][i] = v[k];
+          for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v[k] >> (8 * p)) & 255u], 1u);
+        }
+      }
     }
     __syncthreads();
     int src = 0;
