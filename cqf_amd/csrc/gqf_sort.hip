@@ -292,7 +292,24 @@ struct BucketSmem {
   uint32_t wcnt[kBWaves][256];        // per wave digit counts, then positions
   uint32_t tot[kBMaxPasses][256];     // digit totals of every pass
   uint32_t run[256];                  // next free position of each digit
+  uint32_t oversize;                  // a sub-bucket holds more than 64 keys
 };
+
+// Ascending bitonic sort of one value per lane across the wave.
+__device__ __forceinline__ uint32_t wave_bitonic_sort(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t p = (uint32_t)__shfl_xor((int)v, j, 64);
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == up) ? (v < p ? v : p) : (v > p ? v : p);
+    }
+  }
+  return v;
+}
 
 // Stable scatter of src[0, m) into dst by digit (v >> sh) & 255, given the
 // digit totals; every thread of the workgroup calls it.
@@ -379,7 +396,21 @@ gqf_sort_buckets(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const ui
   for (int k = t; k < kBMaxPasses * 256; k += kBThreads) (&sm.tot[0][0])[k] = 0;
   __syncthreads();
   if (m <= kBucketCap) {
+    // Uniform hashes: scatter by the top (up to) 9 of the low bits into
+    // sub-buckets of ~32 keys (any order: they are split again below), then
+    // each wave sorts whole sub-buckets of at most 64 keys with a bitonic
+    // network across its lanes.  A bucket with a larger sub-bucket (repeated
+    // keys) takes the stable 8-bit passes instead.
     const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
+    const int db = low_bits < 9 ? low_bits : 9;
+    const int sb = low_bits - db;
+    const uint32_t nsub = 1u << db;
+    uint32_t* sub_cnt = &sm.wcnt[0][0];           // nsub counts, then cursors
+    uint32_t* sub_start = &sm.wcnt[0][0] + 1024;  // nsub + 1 starts
+    const int w = t >> 6, lane = t & 63;
+    for (uint32_t d = t; d < nsub; d += kBThreads) sub_cnt[d] = 0;
+    if (t == 0) sm.oversize = 0;
+    __syncthreads();
     for (uint32_t base = 0; base < m; base += kBThreads * kBItems) {
       uint32_t v[kBItems];  // all loads in flight before any is used
 #pragma unroll
@@ -392,16 +423,68 @@ gqf_sort_buckets(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const ui
         const uint32_t i = base + (uint32_t)(k * kBThreads + t);
         if (i < m) {
           sm.buf[0][i] = v[k];
-          for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v[k] >> (8 * p)) & 255u], 1u);
+          atomicAdd(&sub_cnt[v[k] >> sb], 1u);
         }
       }
     }
     __syncthreads();
+    if (w == 0) {  // exclusive scan of the sub-bucket counts, 8 per lane
+      uint32_t c[8], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t d = 8 * lane + k;
+        c[k] = d < nsub ? sub_cnt[d] : 0u;
+        sum += c[k];
+      }
+      uint32_t incl = sum;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += x;
+      }
+      uint32_t r = incl - sum;
+      bool big = false;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t d = 8 * lane + k;
+        if (d < nsub) {
+          sub_start[d] = r;
+          sub_cnt[d] = r;
+          big |= c[k] > 64u;
+        }
+        r += c[k];
+      }
+      if (lane == 63) sub_start[nsub] = r;
+      if (__ballot(big)) sm.oversize = 1;
+    }
+    __syncthreads();
     int src = 0;
-    for (int p = 0; p < passes; p++) {
-      bucket_pass<uint32_t>(sm.buf[src], sm.buf[src ^ 1], m, 8 * p, sm.tot[p], sm);
+    if (!sm.oversize) {
+      for (uint32_t i = t; i < m; i += kBThreads) {
+        const uint32_t v = sm.buf[0][i];
+        sm.buf[1][atomicAdd(&sub_cnt[v >> sb], 1u)] = v;
+      }
       __syncthreads();
-      src ^= 1;
+      for (uint32_t d = w; d < nsub; d += kBThreads / 64) {
+        const uint32_t st = sub_start[d], cnt = sub_start[d + 1] - st;
+        if (cnt == 0) continue;
+        uint32_t x = (uint32_t)lane < cnt ? sm.buf[1][st + lane] : 0xFFFFFFFFu;
+        if (cnt > 1) x = wave_bitonic_sort(x);
+        if ((uint32_t)lane < cnt) sm.buf[1][st + lane] = x;
+      }
+      __syncthreads();
+      src = 1;
+    } else {
+      for (uint32_t i = t; i < m; i += kBThreads) {
+        const uint32_t v = sm.buf[0][i];
+        for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v >> (8 * p)) & 255u], 1u);
+      }
+      __syncthreads();
+      for (int p = 0; p < passes; p++) {
+        bucket_pass<uint32_t>(sm.buf[src], sm.buf[src ^ 1], m, 8 * p, sm.tot[p], sm);
+        __syncthreads();
+        src ^= 1;
+      }
     }
     for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | sm.buf[src][i];
     return;

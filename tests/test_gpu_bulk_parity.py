@@ -303,3 +303,26 @@ def test_two_level_sort_repeated_keys(gpu, oracle):
     finally:
         g.close()
         o.close()
+
+
+def test_two_level_sort_moderate_repeats(gpu, oracle):
+    """2^20 draws over 2^16 keys: buckets fit in LDS, but some of their
+    sub-buckets hold more than 64 keys, so those buckets take the stable
+    8-bit passes instead of the per-wave bitonic sorts."""
+    from cqf_amd.gqf import GQF, to_device_u64, to_numpy_u64
+    nbits = 21
+    _say("case two-level sort, 2^20 draws over 2^16 keys")
+    univ = oracle.splitmix64(45, 1 << 16)
+    keys = univ[oracle.splitmix64(46, (1 << 20) + 777) % np.uint64(univ.size)]
+    g, o = GQF(nbits), oracle.OracleQF(nbits)
+    try:
+        dk = to_device_u64(keys)
+        g.bulk_insert(dk)
+        hk = keys.copy()
+        o.bulk_insert(hk)
+        assert np.array_equal(to_numpy_u64(dk), hk)
+        _assert_images_equal(g.export_image(), o.image(), "image after moderate repeats")
+        assert np.array_equal(to_numpy_u64(g.bulk_get(to_device_u64(univ))), o.bulk_get(univ))
+    finally:
+        g.close()
+        o.close()
