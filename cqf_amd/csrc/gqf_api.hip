@@ -42,6 +42,7 @@ struct Workspace {
   uint64_t* tmp = nullptr;     // sort ping-pong, cap keys
   uint32_t* hist = nullptr;    // [digit][tile]
   uint32_t* totals = nullptr;  // 1024 digit totals
+  uint32_t* slow_buckets = nullptr;  // flags of the two-level sort, kept zero by its kernels
   uint32_t* tile_heads = nullptr;
   uint64_t* ukeys = nullptr;   // run-length reduce outputs, cap keys each
   uint64_t* upos = nullptr;
@@ -102,6 +103,8 @@ void free_workspace(Workspace& w) {
 void init_region_workspace(Workspace& w, uint64_t nblocks) {
   if (!w.totals) {
     dev_alloc(&w.totals, 1024);
+    dev_alloc(&w.slow_buckets, GQF_SORT_MAX_BUCKETS);
+    GQF_CHECK(hipMemset(w.slow_buckets, 0, GQF_SORT_MAX_BUCKETS * sizeof(uint32_t)));
     dev_alloc(&w.m_dev, 1);
     // sized for the smallest tunable region, so retuning never reallocates
     const uint64_t maxreg = (nblocks + GQF_REGION_BLOCKS_MIN - 1) / GQF_REGION_BLOCKS_MIN;
@@ -144,7 +147,8 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   const int vb = (int)f->host.value_bits;
   const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
   if (hash_it && gqf_sort_two_level_fits(n, bits))
-    gqf_launch_sort_two_level(keys, w.tmp, n, gqf_mask(f->host.key_bits), vb, bits, w.hist, w.totals, s);
+    gqf_launch_sort_two_level(keys, w.tmp, n, gqf_mask(f->host.key_bits), vb, bits, w.hist, w.totals,
+                              w.slow_buckets, s);
   else
     gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
                     w.hist, w.totals, s);
@@ -304,6 +308,7 @@ Filter* create_filter(int nbits, int vb) {
 void free_all_workspace(Workspace& w) {
   free_workspace(w);
   dev_free(w.totals);
+  dev_free(w.slow_buckets);
   dev_free(w.m_dev);
   dev_free(w.region_start);
   dev_free(w.region_list);

@@ -18,8 +18,10 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
 // buckets of about 12K keys, then an LDS sort per bucket; not stable on
 // equal hashes (plain keys only: equal hashes are indistinguishable)
 bool gqf_sort_two_level_fits(uint64_t n, int bits);
+// slow: GQF_SORT_MAX_BUCKETS flags, zero between calls (the kernels leave them so)
+#define GQF_SORT_MAX_BUCKETS 1024
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
-                               uint32_t* hist, uint32_t* totals, hipStream_t s);
+                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
 // tile_heads: ceil(n/4096) words; upos: n words.

@@ -292,7 +292,6 @@ struct BucketSmem {
   uint32_t wcnt[kBWaves][256];        // per wave digit counts, then positions
   uint32_t tot[kBMaxPasses][256];     // digit totals of every pass
   uint32_t run[256];                  // next free position of each digit
-  uint32_t oversize;                  // a sub-bucket holds more than 64 keys
 };
 
 // Ascending bitonic sort of one value per lane across the wave.
@@ -382,109 +381,129 @@ __device__ __forceinline__ void bucket_pass(const V* src, V* dst, uint32_t m, in
 
 }  // namespace
 
+// Common case, two workgroups per CU: the bucket's keys stay in registers
+// (17 per thread) while they are counted by sub-bucket -- the top (up to) 9
+// of the low bits, ~32 keys each for uniform hashes -- then go to one LDS
+// buffer in sub-bucket order (any order within: they are split again), and
+// each wave sorts whole sub-buckets of at most 64 keys with a bitonic
+// network across its lanes.  A bucket above kBucketCap keys, or with a
+// sub-bucket above 64 (repeated keys), is flagged for gqf_sort_buckets_slow.
+constexpr int kBLoad = (int)((kBucketCap + kBThreads - 1) / kBThreads);  // 17
+
 extern "C" __global__ void __launch_bounds__(kBThreads)
-gqf_sort_buckets(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
-                 uint32_t nbuckets) {
-  __shared__ BucketSmem sm;
+gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int low_bits,
+                 const uint32_t* __restrict__ starts, uint32_t nbuckets, uint32_t* __restrict__ slow) {
+  __shared__ uint32_t buf[kBucketCap];
+  __shared__ uint32_t sub_cnt[512];    // counts, then cursors
+  __shared__ uint32_t sub_start[513];
+  __shared__ uint32_t oversize;
   const uint32_t b = blockIdx.x;
   const uint64_t s = starts[b];
   const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
   if (e <= s) return;
   const uint32_t m = (uint32_t)(e - s);
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (m > kBucketCap) {
+    if (t == 0) slow[b] = 1;
+    return;
+  }
+  const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
+  const int db = low_bits < 9 ? low_bits : 9;
+  const int sb = low_bits - db;
+  const uint32_t nsub = 1u << db;
+  for (uint32_t d = t; d < nsub; d += kBThreads) sub_cnt[d] = 0;
+  if (t == 0) oversize = 0;
+  uint32_t v[kBLoad];
+#pragma unroll
+  for (int k = 0; k < kBLoad; k++) {
+    const uint32_t i = (uint32_t)(k * kBThreads + t);
+    v[k] = i < m ? (uint32_t)(in[s + i] - hi) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kBLoad; k++)
+    if ((uint32_t)(k * kBThreads + t) < m) atomicAdd(&sub_cnt[v[k] >> sb], 1u);
+  __syncthreads();
+  if (w == 0) {  // exclusive scan of the sub-bucket counts, 8 per lane
+    uint32_t c[8], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t d = 8 * lane + k;
+      c[k] = d < nsub ? sub_cnt[d] : 0u;
+      sum += c[k];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t x = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += x;
+    }
+    uint32_t r = incl - sum;
+    bool big = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t d = 8 * lane + k;
+      if (d < nsub) {
+        sub_start[d] = r;
+        sub_cnt[d] = r;
+        big |= c[k] > 64u;
+      }
+      r += c[k];
+    }
+    if (lane == 63) sub_start[nsub] = r;
+    if (__ballot(big)) oversize = 1;
+  }
+  __syncthreads();
+  if (oversize) {
+    if (t == 0) slow[b] = 1;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kBLoad; k++)
+    if ((uint32_t)(k * kBThreads + t) < m) buf[atomicAdd(&sub_cnt[v[k] >> sb], 1u)] = v[k];
+  __syncthreads();
+  for (uint32_t d = w; d < nsub; d += kBThreads / 64) {
+    const uint32_t st = sub_start[d], cnt = sub_start[d + 1] - st;
+    if (cnt == 0) continue;
+    uint32_t x = (uint32_t)lane < cnt ? buf[st + lane] : 0xFFFFFFFFu;
+    if (cnt > 1) x = wave_bitonic_sort(x);
+    if ((uint32_t)lane < cnt) buf[st + lane] = x;
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | buf[i];
+}
+
+// The flagged buckets (exits at once for the others, and clears the flags
+// it takes): stable 8-bit passes over the bucket's low bits in two LDS
+// buffers, or, for a bucket above kBucketCap, the same passes over its
+// 64-bit keys between its ranges of in and out in global memory.
+extern "C" __global__ void __launch_bounds__(kBThreads)
+gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
+                      uint32_t nbuckets, uint32_t* __restrict__ slow) {
+  __shared__ BucketSmem sm;
+  const uint32_t b = blockIdx.x;
+  if (!slow[b]) return;
+  const uint64_t s = starts[b];
+  const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
+  const uint32_t m = (uint32_t)(e - s);
   const int t = threadIdx.x;
   const int passes = (low_bits + 7) / 8;
   for (int k = t; k < kBMaxPasses * 256; k += kBThreads) (&sm.tot[0][0])[k] = 0;
   __syncthreads();
+  if (t == 0) slow[b] = 0;
   if (m <= kBucketCap) {
-    // Uniform hashes: scatter by the top (up to) 9 of the low bits into
-    // sub-buckets of ~32 keys (any order: they are split again below), then
-    // each wave sorts whole sub-buckets of at most 64 keys with a bitonic
-    // network across its lanes.  A bucket with a larger sub-bucket (repeated
-    // keys) takes the stable 8-bit passes instead.
-    const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
-    const int db = low_bits < 9 ? low_bits : 9;
-    const int sb = low_bits - db;
-    const uint32_t nsub = 1u << db;
-    uint32_t* sub_cnt = &sm.wcnt[0][0];           // nsub counts, then cursors
-    uint32_t* sub_start = &sm.wcnt[0][0] + 1024;  // nsub + 1 starts
-    const int w = t >> 6, lane = t & 63;
-    for (uint32_t d = t; d < nsub; d += kBThreads) sub_cnt[d] = 0;
-    if (t == 0) sm.oversize = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < m; base += kBThreads * kBItems) {
-      uint32_t v[kBItems];  // all loads in flight before any is used
-#pragma unroll
-      for (int k = 0; k < kBItems; k++) {
-        const uint32_t i = base + (uint32_t)(k * kBThreads + t);
-        v[k] = i < m ? (uint32_t)(in[s + i] - hi) : 0u;
-      }
-#pragma unroll
-      for (int k = 0; k < kBItems; k++) {
-        const uint32_t i = base + (uint32_t)(k * kBThreads + t);
-        if (i < m) {
-          sm.buf[0][i] = v[k];
-          atomicAdd(&sub_cnt[v[k] >> sb], 1u);
-        }
-      }
-    }
-    __syncthreads();
-    if (w == 0) {  // exclusive scan of the sub-bucket counts, 8 per lane
-      uint32_t c[8], sum = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const uint32_t d = 8 * lane + k;
-        c[k] = d < nsub ? sub_cnt[d] : 0u;
-        sum += c[k];
-      }
-      uint32_t incl = sum;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t x = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += x;
-      }
-      uint32_t r = incl - sum;
-      bool big = false;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const uint32_t d = 8 * lane + k;
-        if (d < nsub) {
-          sub_start[d] = r;
-          sub_cnt[d] = r;
-          big |= c[k] > 64u;
-        }
-        r += c[k];
-      }
-      if (lane == 63) sub_start[nsub] = r;
-      if (__ballot(big)) sm.oversize = 1;
+    const uint64_t hi = (uint64_t)b << low_bits;
+    for (uint32_t i = t; i < m; i += kBThreads) {
+      const uint32_t v = (uint32_t)(in[s + i] - hi);
+      sm.buf[0][i] = v;
+      for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v >> (8 * p)) & 255u], 1u);
     }
     __syncthreads();
     int src = 0;
-    if (!sm.oversize) {
-      for (uint32_t i = t; i < m; i += kBThreads) {
-        const uint32_t v = sm.buf[0][i];
-        sm.buf[1][atomicAdd(&sub_cnt[v >> sb], 1u)] = v;
-      }
+    for (int p = 0; p < passes; p++) {
+      bucket_pass<uint32_t>(sm.buf[src], sm.buf[src ^ 1], m, 8 * p, sm.tot[p], sm);
       __syncthreads();
-      for (uint32_t d = w; d < nsub; d += kBThreads / 64) {
-        const uint32_t st = sub_start[d], cnt = sub_start[d + 1] - st;
-        if (cnt == 0) continue;
-        uint32_t x = (uint32_t)lane < cnt ? sm.buf[1][st + lane] : 0xFFFFFFFFu;
-        if (cnt > 1) x = wave_bitonic_sort(x);
-        if ((uint32_t)lane < cnt) sm.buf[1][st + lane] = x;
-      }
-      __syncthreads();
-      src = 1;
-    } else {
-      for (uint32_t i = t; i < m; i += kBThreads) {
-        const uint32_t v = sm.buf[0][i];
-        for (int p = 0; p < passes; p++) atomicAdd(&sm.tot[p][(v >> (8 * p)) & 255u], 1u);
-      }
-      __syncthreads();
-      for (int p = 0; p < passes; p++) {
-        bucket_pass<uint32_t>(sm.buf[src], sm.buf[src ^ 1], m, 8 * p, sm.tot[p], sm);
-        __syncthreads();
-        src ^= 1;
-      }
+      src ^= 1;
     }
     for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | sm.buf[src][i];
     return;
@@ -659,7 +678,7 @@ bool gqf_sort_two_level_fits(uint64_t n, int bits) {
 }
 
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
-                               uint32_t* hist, uint32_t* totals, hipStream_t s) {
+                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s) {
   // 2^msd buckets of about 12K keys (at most 1024), low bits <= 32
   int msd = 4;
   while (msd < 10 && (n >> msd) > 12288) msd++;
@@ -675,8 +694,10 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
                      msd, 1, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
   gqf_prof_end("sort_pass", s);
   gqf_prof_begin("sort_buckets", s);
-  hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
-                     (const uint32_t*)totals, (uint32_t)radix);
+  hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, (const uint64_t*)tmp, keys, n, bits - msd,
+                     (const uint32_t*)totals, (uint32_t)radix, slow);
+  hipLaunchKernelGGL(gqf_sort_buckets_slow, dim3(radix), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
+                     (const uint32_t*)totals, (uint32_t)radix, slow);
   gqf_prof_end("sort_buckets", s);
 }
 
