@@ -221,22 +221,7 @@ __device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, 
     tile_start[d] = run;  // tile count, scanned below
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    // exclusive scan of tile_start over `radix` digits by one wave
-    uint32_t carry = 0;
-    for (int b0 = 0; b0 < radix; b0 += 64) {
-      uint32_t v = tile_start[b0 + lane];
-      uint32_t incl = v;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-      }
-      tile_start[b0 + lane] = carry + incl - v;
-      carry += __shfl(incl, 63, 64);
-    }
-  }
-  __syncthreads();
+  block_exclusive_scan(tile_start, radix);  // all waves, 4 digits per thread
 #pragma unroll
   for (int k = 0; k < ITEMS; k++) {
     uint64_t i = base + (uint64_t)w * (64 * ITEMS) + (uint64_t)k * 64 + lane;
