@@ -137,6 +137,46 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
                                                    int32_t st_o, uint32_t old_len, int32_t st_n) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
+  // Common case: one item adding or removing 1 on a run (at most 8 slots) of
+  // count-1 counters.  Such a run's bytes strictly ascend -- a counter of
+  // count >= 2 repeats or drops a byte (r r, r 0 r, 0 0 0, r [0] d.. r,
+  // 0 d.. 0 0) -- and each byte is a remainder.  Adding r puts a byte r before
+  // the first larger byte (r r, if r was there, is its count-2 counter);
+  // removing r drops its byte, or leaves the run as it is when r is absent.
+  // The old bytes are read together, one LDS round trip.
+  if (it1 == it0 + 1 && old_len <= 8) {
+    const int64_t d = src.delta(it0);
+    if (d == 1 || d == -1) {
+      const uint32_t r = src.hash(sm.items, it0) & 0xFFu;
+      uint32_t ob[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) ob[k] = (uint32_t)k < old_len ? in[sin_off(st_o + k)] : 0u;
+      bool simple = true;
+      uint32_t pos = old_len;  // first byte above r
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if ((uint32_t)k < old_len) {
+          if (k && ob[k] <= ob[k - 1]) simple = false;
+          if (pos == old_len && ob[k] > r) pos = (uint32_t)k;
+        }
+      }
+      if (simple) {
+        const bool present = pos > 0 && ob[pos - 1] == r;
+        // new slot of old byte k: +1 from pos on (insert), -1 after a removed byte
+        const int32_t gap = d == 1 ? (int32_t)pos : (present ? (int32_t)pos - 1 : 8);
+        const int32_t step = d == 1 ? 1 : -1;
+        if (WRITE) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            if ((uint32_t)k < old_len && !(d == -1 && k == gap))
+              out[sin_off(st_n + k + (k >= gap ? step : 0))] = (uint8_t)ob[k];
+          }
+          if (d == 1) out[sin_off(st_n + (int32_t)pos)] = (uint8_t)r;
+        }
+        return d == 1 ? old_len + 1 : (present ? old_len - 1 : old_len);
+      }
+    }
+  }
   uint32_t len = 0;
   auto old_slot = [&](uint32_t k) -> uint64_t { return in[sin_off(st_o + (int32_t)k)]; };
   auto put = [&](uint32_t k, uint64_t v) {
