@@ -160,20 +160,25 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
           if (pos == old_len && ob[k] > r) pos = (uint32_t)k;
         }
       }
-      if (simple) {
-        const bool present = pos > 0 && ob[pos - 1] == r;
-        // new slot of old byte k: +1 from pos on (insert), -1 after a removed byte
-        const int32_t gap = d == 1 ? (int32_t)pos : (present ? (int32_t)pos - 1 : 8);
-        const int32_t step = d == 1 ? 1 : -1;
+      if (simple && d == 1) {
         if (WRITE) {
 #pragma unroll
-          for (int k = 0; k < 8; k++) {
-            if ((uint32_t)k < old_len && !(d == -1 && k == gap))
-              out[sin_off(st_n + k + (k >= gap ? step : 0))] = (uint8_t)ob[k];
-          }
-          if (d == 1) out[sin_off(st_n + (int32_t)pos)] = (uint8_t)r;
+          for (int k = 0; k < 8; k++)
+            if ((uint32_t)k < old_len) out[sin_off(st_n + k + ((uint32_t)k >= pos ? 1 : 0))] = (uint8_t)ob[k];
+          out[sin_off(st_n + (int32_t)pos)] = (uint8_t)r;
         }
-        return d == 1 ? old_len + 1 : (present ? old_len - 1 : old_len);
+        return old_len + 1;
+      }
+      if (simple) {
+        const bool present = pos > 0 && ob[pos - 1] == r;
+        const uint32_t gone = present ? pos - 1 : 8u;  // the removed byte
+        if (WRITE) {
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if ((uint32_t)k < old_len && (uint32_t)k != gone)
+              out[sin_off(st_n + k - ((uint32_t)k > gone ? 1 : 0))] = (uint8_t)ob[k];
+        }
+        return present ? old_len - 1 : old_len;
       }
     }
   }
