@@ -74,7 +74,9 @@ __device__ inline int lane_id() { return (int)__lane_id(); }
 // byte offset of window-relative slot p in the slot rows (sin / sout): one
 // 68-byte row per block, so lanes reading "their" block at the same column
 // hit different LDS banks
-__device__ inline uint32_t sin_off(int32_t p) { return (uint32_t)(p >> 6) * 68 + (p & 63); }
+__device__ inline uint32_t sin_off(int32_t p) {
+  return (uint32_t)p + ((uint32_t)(p >> 6) << 2);  // = 68 (p >> 6) + (p & 63), for p >= 0
+}
 
 __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   // (a, b) then (a2, b2): x -> max(a2, max(a, x + b) + b2)
