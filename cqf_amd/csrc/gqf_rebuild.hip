@@ -318,8 +318,9 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
 
 // ---------------------------------------------------------------------------
 // Fast path (all of a region's items cached in LDS).  The quotients the
-// items touch are listed once (table tq/tit0); each lane then locates the old
-// run of one listed quotient straight from its block's header -- the rank
+// items touch are listed once (table tit0: each quotient's first item);
+// each lane then locates the old run of one listed quotient straight from
+// its block's header -- the rank
 // of the quotient among the block's occupied quotients selects its run end
 // among the run ends after the block's end-before -- and merges it with the
 // quotient's items.  The per-block walk then only reads the table, so its
