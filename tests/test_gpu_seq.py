@@ -134,3 +134,40 @@ def test_seq_rejects_bad_requests(gpu):
         assert not g.export_image().any()  # nothing was applied
     finally:
         g.close()
+
+
+def test_seq_batch_with_value_bits(gpu, oracle):
+    """A filter with 2 value bits (the layout a resize leaves): slot hash =
+    hash_64(key under 2^key_bits) << 2.  The expected counts apply the
+    requests one at a time to a {slot hash: count} map (deletes floor at 0);
+    the expected image is the canonical layout of the final map."""
+    from cqf_amd.gqf import GQF, to_device_u64
+    nbits, vb = 14, 2
+    key_bits = nbits + 8 - vb
+    _say(f"case seq with value bits q={nbits} vb={vb}")
+    rng = np.random.default_rng(9)
+    univ = oracle.splitmix64(109, 3000)
+    reqs = _requests(oracle, rng, univ, 60, 300, p_get=0.4, p_del=0.3)
+    slot = lambda k: oracle.hash_many(k, (1 << key_bits) - 1) << np.uint64(vb)
+    counts, want = {}, {}
+    for i, (op, k) in enumerate(reqs):
+        hs = slot(k)
+        if op == "get":
+            want[i] = np.array([counts.get(int(h), 0) for h in hs], dtype=np.uint64)
+        else:
+            for h in hs.tolist():
+                c = counts.get(h, 0)
+                counts[h] = c + 1 if op == "insert" else max(0, c - 1)
+    g = GQF(nbits, value_bits=vb)
+    try:
+        assert g.info["value_bits"] == vb
+        got = _run_seq(g, reqs, gpu)
+        assert g.status() == 0
+        for i in want:
+            assert np.array_equal(got[i], want[i]), f"request {i}"
+        live = {h: c for h, c in counts.items() if c}
+        hs = np.array(sorted(live), dtype=np.uint64)
+        cs = np.array([live[int(h)] for h in hs], dtype=np.uint64)
+        assert np.array_equal(g.export_image(), oracle.canonical_image(nbits, hs, cs))
+    finally:
+        g.close()
