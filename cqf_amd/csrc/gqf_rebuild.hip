@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kWinFast = 64;          // region blocks + push blocks
 constexpr int kWinWide = 256;         // fallback window (208 push blocks)
-constexpr int kItemCap = 320;         // items cached in LDS per region (fast path)
+constexpr int kItemCap = 248;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
 enum RebuildMode : int { kInsert = 0, kDelete = 1, kSeq = 2 };
@@ -52,7 +52,13 @@ struct Smem {
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
   uint64_t occ_in[W];
-  uint32_t items[kItemCap];  // hash - first hash of the region
+  union {
+    uint32_t items[kItemCap];  // fast path: hash - first hash of the region
+    struct {
+      uint32_t cnt[64];          // general path (items not cached): items per region block
+      uint32_t start[64];        // and the index of each block's first item
+    } blk;
+  };
   // one entry per distinct quotient among the items (fast path); the
   // quotient itself is items[tit0[t]] >> 8
   uint16_t tit0[kItemCap];   // first item of the quotient
@@ -60,8 +66,6 @@ struct Smem {
   uint16_t tol[kItemCap];    // old run length (0: quotient was empty)
   uint16_t tnl[kItemCap];    // new run length
   int16_t tsn[kItemCap];     // new run start
-  uint32_t blk_cnt[64];
-  uint32_t blk_start[64];
   uint32_t nq;
   int32_t first_invalid;
   int32_t xend;
@@ -485,7 +489,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
         if (kind == kCounts || kind == kTagged) lcnt[i] = src.cache_delta(s + i);
       }
     }
-    sm.blk_cnt[lane] = 0;
+    if (!src.cached) sm.blk.cnt[lane] = 0;
     if (lane == 0) { sm.first_invalid = nwin; sm.bstar = -1; sm.result = kOk; }
 #pragma unroll
     for (int k = 0; k < kRounds; k++) {
@@ -525,25 +529,29 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       if (head) {
         const uint32_t t = carry + (uint32_t)__popcll(hm & lt);
         sm.tit0[t] = (uint16_t)i;
-        atomicAdd(&sm.blk_cnt[h >> 14], 1u);
         atomicOr((unsigned long long*)&sm.tmask[h >> 14], 1ULL << ((h >> 8) & 63));
       }
       carry += (uint32_t)__popcll(hm);
     }
     if (lane == 0) sm.nq = carry;
   } else {
-    for (uint32_t i = lane; i < m; i += 64) atomicAdd(&sm.blk_cnt[src.hash(sm.items, i) >> 14], 1u);
+    for (uint32_t i = lane; i < m; i += 64) atomicAdd(&sm.blk.cnt[src.hash(sm.items, i) >> 14], 1u);
   }
   __syncthreads();
+  // block `lane` (a region block: there are at most 56) starts at table entry
+  // tbase (fast path: its touched quotients, counted by popcount) or at item
+  // blk.start[lane] (general path)
+  uint32_t tbase;
   {
-    const uint32_t c = sm.blk_cnt[lane];
+    const uint32_t c = fast ? (lane < nreg ? (uint32_t)__popcll(sm.tmask[lane]) : 0u) : sm.blk.cnt[lane];
     uint32_t incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       uint32_t t = __shfl_up(incl, d, 64);
       if (lane >= d) incl += t;
     }
-    sm.blk_start[lane] = incl - c;
+    tbase = incl - c;
+    if (!fast) sm.blk.start[lane] = tbase;
   }
   if (fast) {
     // old run and new length of every listed quotient, one quotient per lane
@@ -581,8 +589,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       if (j < nwin) {
         uint32_t it = 0, ie = 0;
         if (j < nreg) {
-          it = sm.blk_start[j];
-          ie = it + sm.blk_cnt[j];
+          it = fast ? tbase : sm.blk.start[j];
+          ie = fast ? 0u : it + sm.blk.cnt[j];
         }
         const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], a, b, occ_new[k])
                              : walk_block<W, false>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occ_new[k]);
@@ -674,8 +682,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     if (j < nrelay) {
       uint32_t it = 0, ie = 0;
       if (j < nreg) {
-        it = sm.blk_start[j];
-        ie = it + sm.blk_cnt[j];
+        it = fast ? tbase : sm.blk.start[j];
+        ie = fast ? 0u : it + sm.blk.cnt[j];
       }
       int32_t a, b;
       uint64_t occn;
