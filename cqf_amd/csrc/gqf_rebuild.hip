@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kWinFast = 64;          // region blocks + push blocks
 constexpr int kWinWide = 256;         // fallback window (208 push blocks)
-constexpr int kItemCap = 248;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
+constexpr int kItemCap = 232;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
 enum RebuildMode : int { kInsert = 0, kDelete = 1, kSeq = 2 };
@@ -47,11 +47,18 @@ struct Smem {
   uint32_t sin[W * 17];   // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
   uint32_t sout[W * 17];  // new slots, same rows
   uint64_t run_in[W];
-  uint64_t run_out[W];
-  uint64_t tmask[W];     // quotients of block j touched by the items (fast path)
+  // arrays whose lifetimes do not overlap share space
+  union {
+    uint64_t occ_in[W];    // old occupieds, until the quotient table is built
+    uint64_t run_out[W];   // new run ends, from pass 2 on
+  };
+  union {
+    uint64_t tmask[W];         // quotients of block j touched by the items, until
+                               // the walks take them into registers (fast path)
+    int16_t tsn[kItemCap];     // new run start of each listed quotient, from pass 2 on
+  };
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
-  uint64_t occ_in[W];
   union {
     uint32_t items[kItemCap];  // fast path: hash - first hash of the region
     struct {
@@ -65,7 +72,6 @@ struct Smem {
   int16_t tst[kItemCap];     // old run start
   uint16_t tol[kItemCap];    // old run length (0: quotient was empty)
   uint16_t tnl[kItemCap];    // new run length
-  int16_t tsn[kItemCap];     // new run start
   uint32_t nq;
   int32_t first_invalid;
   int32_t xend;
@@ -366,10 +372,10 @@ __device__ __forceinline__ bool locate_old_run(const Smem<W>& sm, int nwin, int3
 
 template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tbase, uint64_t occ,
-                                                 int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
+                                                 uint64_t touched, int32_t& agg_a, int32_t& agg_b,
+                                                 uint64_t& newocc) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
-  const uint64_t touched = sm.tmask[j];
   const int32_t blk_base = 64 * j;
   int32_t xo = sm.xold[j];
   int32_t xn = WRITE ? sm.xnew[j] : 0;
@@ -553,6 +559,14 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     tbase = incl - c;
     if (!fast) sm.blk.start[lane] = tbase;
   }
+  // the touched quotients of this lane's blocks, in registers from here on
+  // (the tmask space becomes tsn in pass 2)
+  uint64_t tm_w[kRounds];
+#pragma unroll
+  for (int k = 0; k < kRounds; k++) {
+    const int j = lane + 64 * k;
+    tm_w[k] = fast && j < nwin ? sm.tmask[j] : 0ULL;
+  }
   if (fast) {
     // old run and new length of every listed quotient, one quotient per lane
     const uint32_t nq = sm.nq;
@@ -592,7 +606,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
           it = fast ? tbase : sm.blk.start[j];
           ie = fast ? 0u : it + sm.blk.cnt[j];
         }
-        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], a, b, occ_new[k])
+        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occ_new[k])
                              : walk_block<W, false>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occ_new[k]);
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
@@ -687,7 +701,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       }
       int32_t a, b;
       uint64_t occn;
-      if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], a, b, occn);
+      if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occn);
       else walk_block<W, true>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occn);
     }
   }
