@@ -1077,8 +1077,8 @@ uint32_t gqf_group_rebuild_grid() {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gqf_group_rebuild, 64, 0);
     if (per_cu < 1) per_cu = 1;
-    // LDS would allow 10 (16 KB each of 160 KB); 8 measured faster for the
-    // 256-filter group insert (162 ms against 195 ms)
+    // LDS would allow 12 (13.25 KB each of 160 KB); 8 measured faster for
+    // the 256-filter group insert (156 ms against 172 ms with 12)
     if (per_cu > 8) per_cu = 8;
     grid = (uint32_t)((cus > 0 ? cus : 1) * per_cu);
   }
