@@ -458,16 +458,9 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
   for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | buf[i];
 }
 
-// The flagged buckets (exits at once for the others, and clears the flags
-// it takes): stable 8-bit passes over the bucket's low bits in two LDS
-// buffers, or, for a bucket above kBucketCap, the same passes over its
-// 64-bit keys between its ranges of in and out in global memory.
-extern "C" __global__ void __launch_bounds__(kBThreads)
-gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
-                      uint32_t nbuckets, uint32_t* __restrict__ slow) {
-  __shared__ BucketSmem sm;
-  const uint32_t b = blockIdx.x;
-  if (!slow[b]) return;
+__device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out, uint64_t n, int low_bits,
+                                                 const uint32_t* __restrict__ starts, uint32_t nbuckets,
+                                                 uint32_t* __restrict__ slow, uint32_t b, BucketSmem& sm) {
   const uint64_t s = starts[b];
   const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
   const uint32_t m = (uint32_t)(e - s);
@@ -510,6 +503,22 @@ gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, con
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+}
+
+// The flagged buckets (skips the others, and clears the flags it takes):
+// stable 8-bit passes over the bucket's low bits in two LDS buffers, or, for
+// a bucket above kBucketCap, the same passes over its 64-bit keys between
+// its ranges of in and out in global memory.  A grid of at most one
+// workgroup per CU strides over the buckets, so an all-clear call costs
+// one short launch.
+extern "C" __global__ void __launch_bounds__(kBThreads)
+gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
+                      uint32_t nbuckets, uint32_t* __restrict__ slow) {
+  __shared__ BucketSmem sm;
+  for (uint32_t b = blockIdx.x; b < nbuckets; b += gridDim.x) {
+    if (slow[b]) sort_flagged_bucket(in, out, n, low_bits, starts, nbuckets, slow, b, sm);
+    __syncthreads();
   }
 }
 
@@ -681,7 +690,7 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
   gqf_prof_begin("sort_buckets", s);
   hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, (const uint64_t*)tmp, keys, n, bits - msd,
                      (const uint32_t*)totals, (uint32_t)radix, slow);
-  hipLaunchKernelGGL(gqf_sort_buckets_slow, dim3(radix), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
+  hipLaunchKernelGGL(gqf_sort_buckets_slow, dim3(radix < 256 ? radix : 256), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
                      (const uint32_t*)totals, (uint32_t)radix, slow);
   gqf_prof_end("sort_buckets", s);
 }
