@@ -35,7 +35,8 @@
 namespace {
 
 constexpr int kWinFast = 64;          // region blocks + push blocks
-constexpr int kWinWide = 256;         // fallback window (208 push blocks)
+constexpr int kWinMid = 128;          // first fallback window (80 push blocks)
+constexpr int kWinWide = 256;         // last fallback window (208 push blocks)
 constexpr int kItemCap = 232;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
@@ -803,13 +804,16 @@ gqf_rebuild_phase_counts(const QF* __restrict__ qfp, const uint64_t* __restrict_
                      sm, icnt);
 }
 
-// Redoes flagged regions of one parity in ascending order with the wide
-// window.  One workgroup; exits at once when nothing was flagged.
+// Redoes flagged regions of one parity in ascending order with a wider
+// window: 128 blocks, then 256 for a push that outruns that too (a region
+// that needs a wider window has not written anything).  One workgroup;
+// exits at once when nothing was flagged.
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                  const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
                  int parity, int mode, uint8_t* __restrict__ wide_flags,
                  uint32_t* __restrict__ wide_count) {
+  __shared__ Smem<kWinMid> smm;
   __shared__ Smem<kWinWide> sm;
   __shared__ uint32_t icnt[kItemCap];
   if (*(volatile uint32_t*)wide_count == 0) return;
@@ -824,7 +828,9 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
       m &= m - 1;
       const uint64_t rr = base + 2ULL * l;
       const uint64_t s = region_start[rr], e = region_start[rr + 1];
-      const int res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
+      int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt);
+      __syncthreads();
+      if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
       __syncthreads();
       if (lane == 0) {
         wide_flags[rr] = 0;
@@ -1000,6 +1006,7 @@ gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode)
 // ascending region order.  One workgroup; exits at once when none.
 extern "C" __global__ void __launch_bounds__(64)
 gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
+  __shared__ Smem<kWinMid> smm;
   __shared__ Smem<kWinWide> sm;
   if (*(volatile uint32_t*)&g.counters[2] == 0) return;
   const int lane = lane_id();
@@ -1014,7 +1021,9 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
         m &= m - 1;
         uint64_t s, e, tag;
         region_hash_bounds(g, qf, f, rr, items, s, e, tag);
-        const int res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, nullptr, tag);
+        int res = rebuild_region<kWinMid>(qf, rr, items, nullptr, s, e, mode, smm, nullptr, tag);
+        __syncthreads();
+        if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, nullptr, tag);
         __syncthreads();
         if (lane == 0) {
           g.wide[f0 + rr] = 0;
