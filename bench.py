@@ -282,6 +282,7 @@ def groups_main(args):
         "insert_call_ms": round(float(np.mean(ins)), 3), "query_call_ms": round(float(np.mean(qry)), 3),
         "queries_mops": round(n * args.steps / q_s / 1e6, 3),
         "autotune_s": round(t_tune, 2), "tuning_region_blocks/digit": tuning,
+        "workgroups_per_cu": group.workgroups_per_cu,
         "keygen_s": round(t_gen, 2),
         "checks": {"all_found": all_found, "exact_counts_16_filters": exact,
                    "status_nonzero": int(sum(1 for x in status if x))},

@@ -119,6 +119,7 @@ struct Group {
   uint32_t* hist = nullptr;
   uint32_t* totals = nullptr;
   uint64_t cap = 0;
+  uint32_t per_cu = 0;  // workgroups per CU of the persistent rebuild (0: default)
   // graph cache
   bool use_graph = true;
   hipStream_t cap_stream = nullptr;
@@ -215,7 +216,7 @@ void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t
   while ((1ULL << fbits) < dev.nf) fbits++;
   gqf_launch_sort(keys, g->tmp, n, 0, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
   hipLaunchKernelGGL(gqf_group_reset, dim3(1), dim3(64), 0, s, g->d_counters);
-  gqf_launch_group_rebuild(dev, keys, mode, gqf_group_rebuild_grid(), s);
+  gqf_launch_group_rebuild(dev, keys, mode, gqf_group_rebuild_grid(g->per_cu), s);
   hipLaunchKernelGGL(gqf_group_untag, dim3(flat_grid(n)), dim3(256), 0, s, keys, n, dev.tag_shift);
 }
 
@@ -298,7 +299,7 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   ensure_sort_space(g, n);
   if (!g->tmp || !g->hist) return -3;
   hipStream_t s = (hipStream_t)stream;
-  (void)gqf_group_rebuild_grid();  // occupancy query outside any capture
+  (void)gqf_group_rebuild_grid(g->per_cu);  // occupancy query outside any capture
   if (!g->use_graph) {
     enqueue_insert(g, dev, keys, n, is_hash, 0, s);
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -308,6 +309,7 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   key.push_back((uint64_t)is_hash);
   key.push_back((uint64_t)(uintptr_t)g->tmp);
   key.push_back((uint64_t)(uintptr_t)g->d_done);
+  key.push_back((uint64_t)g->per_cu);
   for (const QF& d : g->h_descs) key.push_back(d.region_blocks);
   if (!g->exec || key != g->graph_key) {
     if (g->exec) {
@@ -422,21 +424,127 @@ extern "C" int gqf_autotune(int nbits, double load, uint32_t* region_blocks, uin
   return 0;
 }
 
+namespace {
+
+// Workgroups per CU of the group's persistent rebuild: times the group
+// insert of a batch cut like `offsets` into scratch filters of the same
+// sizes and region size, for each candidate up to what the GPU keeps
+// resident, and returns the fastest (0 if nothing could be timed).
+uint32_t tune_per_cu(Group* g, const uint64_t* offsets) {
+  const uint64_t n = offsets[g->nf];
+  const uint32_t mx = gqf_group_rebuild_max_per_cu();
+  if (n == 0) return 0;
+  std::vector<QF*> scratch(g->nf, nullptr);
+  bool ok = true;
+  for (uint32_t f = 0; f < g->nf && ok; f++) {
+    QF d;
+    ok = gqf_filter_host(g->handles[f], &d);
+    if (!ok) break;
+    qf_malloc_device_kv(&scratch[f], (int)(d.key_bits + d.value_bits) - 8, (int)d.value_bits);
+    ok = scratch[f] && qf_set_tuning(scratch[f], d.region_blocks, 0) == 0;
+  }
+  gqf_group* sh = nullptr;
+  uint64_t* src = nullptr;
+  uint64_t* work = nullptr;
+  if (ok) ok = gqf_group_create(&sh, scratch.data(), g->nf) == 0;
+  if (ok) {
+    dalloc(&src, n);
+    dalloc(&work, n);
+    ok = src && work;
+  }
+  uint32_t best_pc = 0;
+  if (ok) {
+    Group* sg = glookup(sh);
+    sg->use_graph = false;
+    gqf_fill_splitmix64(src, n, 0x9C0Fu + n, 0, nullptr);
+    hipEvent_t a, b;
+    GCHECK(hipEventCreate(&a));
+    GCHECK(hipEventCreate(&b));
+    static const uint32_t kPerCu[] = {4, 6, 8, 10, 12, 16};
+    float best = 1e30f;
+    for (uint32_t pc : kPerCu) {
+      if (pc > mx) break;
+      sg->per_cu = pc;
+      float ms_sum = 0;
+      bool full = false;
+      for (int rep = 0; rep < 2; rep++) {
+        for (QF* q : scratch) qf_reset_device(q, nullptr);
+        GCHECK(hipMemcpy(work, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice));
+        GCHECK(hipEventRecord(a, nullptr));
+        const int rc = gqf_group_bulk_insert(sh, work, offsets, 0, nullptr);
+        GCHECK(hipEventRecord(b, nullptr));
+        GCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        GCHECK(hipEventElapsedTime(&ms, a, b));
+        if (rc != 0) full = true;
+        if (rep) ms_sum += ms;  // the first run warms up
+      }
+      for (QF* q : scratch) {
+        if (qf_get_status(q) != 0) full = true;
+        qf_clear_status(q);
+      }
+      if (!full && ms_sum < best) {
+        best = ms_sum;
+        best_pc = pc;
+      }
+    }
+    GCHECK(hipEventDestroy(a));
+    GCHECK(hipEventDestroy(b));
+  }
+  if (sh) gqf_group_destroy(sh);
+  dfree(src);
+  dfree(work);
+  for (QF* q : scratch)
+    if (q) qf_destroy_device(q);
+  return best_pc;
+}
+
+}  // namespace
+
 // Tunes every filter of the group for the load its segment brings
-// (offsets as for gqf_group_bulk_insert); synchronises.
+// (offsets as for gqf_group_bulk_insert), then the group's workgroups per
+// CU for that batch shape; synchronises.
 extern "C" int gqf_group_autotune(gqf_group* h, const uint64_t* offsets) {
   Group* g = glookup(h);
   if (!g || !offsets) return -1;
+  if (offsets[0] != 0) return -2;
   for (uint32_t f = 0; f < g->nf; f++) {
     QF d;
     if (!gqf_filter_host(g->handles[f], &d)) return -2;
+    if (offsets[f + 1] < offsets[f]) return -2;
     double load = (double)(offsets[f + 1] - offsets[f]) / (double)d.nslots;
     if (load <= 0) continue;
     if (load > 1.0) load = 1.0;
     uint32_t r = 0, dg = 0;
-    int rc = gqf_autotune((int)(d.key_bits - 8), load, &r, &dg);
+    int rc = gqf_autotune((int)(d.key_bits + d.value_bits) - 8, load, &r, &dg);
     if (rc != 0) return rc;
     if (qf_set_tuning(g->handles[f], r, dg) != 0) return -3;
   }
+  const uint32_t pc = tune_per_cu(g, offsets);
+  if (pc) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->per_cu = pc;
+  }
+  return 0;
+}
+
+// Workgroups per CU the group's persistent rebuild launches with: 0 sets
+// the default; above what the GPU keeps resident -> -2.
+extern "C" int gqf_group_set_workgroups_per_cu(gqf_group* h, uint32_t per_cu) {
+  Group* g = glookup(h);
+  if (!g) return -1;
+  if (per_cu > gqf_group_rebuild_max_per_cu()) return -2;
+  std::lock_guard<std::mutex> lk(g->mu);
+  g->per_cu = per_cu;
+  return 0;
+}
+
+extern "C" int gqf_group_get_workgroups_per_cu(gqf_group* h, uint32_t* per_cu) {
+  Group* g = glookup(h);
+  if (!g || !per_cu) return -1;
+  std::lock_guard<std::mutex> lk(g->mu);
+  const uint32_t mx = gqf_group_rebuild_max_per_cu();
+  const uint32_t pc = g->per_cu ? g->per_cu : GQF_GROUP_PER_CU;
+  *per_cu = pc < mx ? pc : mx;
   return 0;
 }

@@ -146,7 +146,9 @@ struct gqf_group_dev {
 // items that wait for their even neighbours), then the wide redo tail
 void gqf_launch_group_rebuild(const gqf_group_dev& g, const uint64_t* items, int mode, uint32_t grid,
                               hipStream_t s);
-uint32_t gqf_group_rebuild_grid();
+#define GQF_GROUP_PER_CU 8  // workgroups per CU of the persistent rebuild unless tuned
+uint32_t gqf_group_rebuild_max_per_cu();
+uint32_t gqf_group_rebuild_grid(uint32_t per_cu);
 // returns[i] = count of vals[i] in the filter whose segment holds i
 void gqf_launch_group_get(const gqf_group_dev& g, const uint64_t* vals, uint64_t n, uint64_t* returns,
                           hipStream_t s);

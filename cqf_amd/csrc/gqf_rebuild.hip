@@ -1076,22 +1076,33 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
   }
 }
 
-// Workgroups the GPU holds at once for the persistent rebuild (all of the
-// grid must be resident: see gqf_group_rebuild).
-uint32_t gqf_group_rebuild_grid() {
-  static uint32_t grid = 0;
-  if (!grid) {
-    int dev = 0, cus = 0, per_cu = 0;
+// Workgroups per CU the persistent rebuild can keep resident (all of its
+// grid must be: see gqf_group_rebuild), and the grid for `per_cu` of them
+// (0: the default, clamped to what is resident).  The default is 8: LDS
+// would allow 12 (13.25 KB each of 160 KB), but 8 measured faster for the
+// 256-filter group insert (156 ms against 172 ms with 12); gqf_group_autotune
+// times the candidates for a group's own shape.
+uint32_t gqf_group_rebuild_max_per_cu() {
+  static int per_cu = 0;
+  if (!per_cu) {
+    int p = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, (const void*)gqf_group_rebuild, 64, 0);
+    per_cu = p < 1 ? 1 : p;
+  }
+  return (uint32_t)per_cu;
+}
+
+uint32_t gqf_group_rebuild_grid(uint32_t per_cu) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)gqf_group_rebuild, 64, 0);
-    if (per_cu < 1) per_cu = 1;
-    // LDS would allow 12 (13.25 KB each of 160 KB); 8 measured faster for
-    // the 256-filter group insert (156 ms against 172 ms with 12)
-    if (per_cu > 8) per_cu = 8;
-    grid = (uint32_t)((cus > 0 ? cus : 1) * per_cu);
+    if (cus < 1) cus = 1;
   }
-  return grid;
+  if (per_cu == 0) per_cu = GQF_GROUP_PER_CU;
+  const uint32_t mx = gqf_group_rebuild_max_per_cu();
+  return (uint32_t)cus * (per_cu < mx ? per_cu : mx);
 }
 
 void gqf_launch_group_rebuild(const gqf_group_dev& g, const uint64_t* items, int mode, uint32_t grid,

@@ -39,6 +39,7 @@ EXPORTS = [
     "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
+    "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
     "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
     "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust",
@@ -124,6 +125,10 @@ def load_library(path=LIB_PATH):
     L.gqf_autotune.restype = ctypes.c_int
     L.gqf_group_autotune.argtypes = [_vp, ctypes.POINTER(_u64)]
     L.gqf_group_autotune.restype = ctypes.c_int
+    L.gqf_group_set_workgroups_per_cu.argtypes = [_vp, ctypes.c_uint32]
+    L.gqf_group_set_workgroups_per_cu.restype = ctypes.c_int
+    L.gqf_group_get_workgroups_per_cu.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
+    L.gqf_group_get_workgroups_per_cu.restype = ctypes.c_int
     L.qf_malloc_device_kv.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
     L.qf_enumerate.argtypes = [_vp, _u64, _u64, ctypes.c_int, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp]
     L.qf_enumerate.restype = ctypes.c_int
@@ -460,6 +465,21 @@ class GQFGroup:
             raise RuntimeError(f"gqf_group_autotune rc={rc}")
         for f in self.filters:
             f.refresh_info()
+
+    @property
+    def workgroups_per_cu(self):
+        """Workgroups per CU of the group's persistent rebuild."""
+        v = ctypes.c_uint32()
+        rc = self._lib.gqf_group_get_workgroups_per_cu(self.handle, ctypes.byref(v))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_get_workgroups_per_cu rc={rc}")
+        return v.value
+
+    @workgroups_per_cu.setter
+    def workgroups_per_cu(self, per_cu):
+        rc = self._lib.gqf_group_set_workgroups_per_cu(self.handle, int(per_cu))
+        if rc != 0:
+            raise ValueError(f"gqf_group_set_workgroups_per_cu({per_cu}) rc={rc}")
 
     def close(self):
         if self.handle is not None and self.handle.value:

@@ -188,8 +188,15 @@ int gqf_group_set_graphs(gqf_group *g, int on);
  * fastest (cached per nbits and load rounded to 0.05); synchronises */
 int gqf_autotune(int nbits, double load, uint32_t *region_blocks, uint32_t *sort_digit_bits);
 /* gqf_autotune for every filter of the group at the load its segment
- * brings, applied with qf_set_tuning */
+ * brings, applied with qf_set_tuning; then the workgroups per CU of the
+ * group's persistent rebuild, timed for a batch cut like `offsets` on
+ * scratch filters of the same sizes (candidates 4..16 up to what the GPU
+ * keeps resident); synchronises */
 int gqf_group_autotune(gqf_group *g, const uint64_t *offsets);
+/* workgroups per CU of the persistent rebuild (0: the default, 8); above
+ * what the GPU keeps resident -> -2.  The getter reports the value used. */
+int gqf_group_set_workgroups_per_cu(gqf_group *g, uint32_t per_cu);
+int gqf_group_get_workgroups_per_cu(gqf_group *g, uint32_t *per_cu);
 
 /* ---- whole-table operations ------------------------------------- */
 

@@ -159,6 +159,7 @@ def test_autotune_choices_keep_results(gpu, oracle):
     group = gqf.GQFGroup(filters)
     try:
         group.autotune(offsets)
+        assert 1 <= group.workgroups_per_cu <= 16
         group.bulk_insert(keys.clone().cuda(), offsets)
         torch.cuda.synchronize()
         allk = keys.numpy().view(np.uint64)
@@ -167,5 +168,41 @@ def test_autotune_choices_keep_results(gpu, oracle):
             o.bulk_insert(allk[offsets[f]:offsets[f + 1]].copy())
             assert np.array_equal(filters[f].export_image(), o.image()), f
             o.close()
+    finally:
+        _close(filters, group)
+
+
+def test_group_workgroups_per_cu_keep_results(gpu, oracle):
+    """The persistent rebuild's grid (workgroups per CU) changes which
+    workgroup takes which region, never the result: 1 per CU (each
+    workgroup loops over many regions), 3, and the most the GPU keeps
+    resident, replayed from graphs, against the oracle."""
+    from cqf_amd.gqf import GQFGroup
+    params, keys, offsets, filters = _setup(10, 12, 15, seed=21)
+    _say(f"case group workgroups per CU 10 filters keys={keys.numel()}")
+    group = GQFGroup(filters)
+    try:
+        assert group.workgroups_per_cu == 8  # the default, when the GPU keeps 8 resident
+        with pytest.raises(ValueError):
+            group.workgroups_per_cu = 1000
+        allk = keys.numpy().view(np.uint64)
+        want = []
+        for f, (q, *_r) in enumerate(params):
+            o = oracle.OracleQF(q)
+            o.bulk_insert(allk[offsets[f]:offsets[f + 1]].copy())
+            want.append(o.image())
+            o.close()
+        dk = torch.empty_like(keys, device="cuda")
+        for pc in (1, 3, 12):
+            group.workgroups_per_cu = pc
+            assert group.workgroups_per_cu <= pc
+            for f in filters:
+                f.reset()
+            dk.copy_(keys.cuda())
+            group.bulk_insert(dk, offsets)
+            torch.cuda.synchronize()
+            for f in range(len(filters)):
+                assert np.array_equal(filters[f].export_image(), want[f]), (pc, f)
+                assert filters[f].status() == 0
     finally:
         _close(filters, group)
