@@ -173,7 +173,10 @@ __device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, 
                                                int vshift, const uint32_t* __restrict__ hist,
                                                const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
   constexpr int TILE = kThreads * ITEMS;
-  __shared__ uint32_t wcnt[kWaves][kMaxRadix];
+  static_assert(TILE < 65536, "per-wave digit counts are 16-bit");
+  // 16-bit per-wave counts keep the kernel at 48 KB of LDS: three
+  // workgroups per CU instead of two
+  __shared__ uint16_t wcnt[kWaves][kMaxRadix];
   __shared__ uint32_t tile_start[kMaxRadix];
   __shared__ uint32_t glob[kMaxRadix];
   __shared__ uint64_t stage[TILE];
@@ -206,7 +209,7 @@ __device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, 
     rank[k] = before + (uint32_t)__popcll(peers & lt_mask);
     keys[k] = key;
     // lowest peer bumps the wave's digit counter
-    if (valid && (peers & lt_mask) == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    if (valid && (peers & lt_mask) == 0) wcnt[w][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
   }
   __syncthreads();
   // prefix over waves per digit, tile totals -> tile_start (exclusive over digits)
@@ -215,7 +218,7 @@ __device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, 
 #pragma unroll
     for (int k = 0; k < kWaves; k++) {
       uint32_t t = wcnt[k][d];
-      wcnt[k][d] = run;
+      wcnt[k][d] = (uint16_t)run;
       run += t;
     }
     tile_start[d] = run;  // tile count, scanned below
