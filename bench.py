@@ -167,12 +167,14 @@ def zipf_main(args):
         step(recs)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern = {k: gqf.profile_read(k) for k in ("sort_pass", "sort_buckets", "region_bounds", "rebuild_phase",
-                                              "rebuild_wide", "bulk_get")}
+    kern = {k: gqf.profile_read(k) for k in ("reduce_count", "reduce_finish", "sort_pass", "sort_buckets",
+                                              "region_bounds", "rebuild_phase", "rebuild_wide", "bulk_get")}
     gqf.profile_enable(False)
     ins = [e[0].elapsed_time(e[1]) for e in recs]
     qry = [e[1].elapsed_time(e[2]) for e in recs]
     exact = bool(torch.equal(ret, mult))
+    # the call leaves `work` holding the ascending hashes of all draws
+    keys_sorted = bool((work[1:] >= work[:-1]).all().item())
     distinct = int((mult > 0).sum().item())
     ins_s = sum(ins) / 1e3
     rate = n_draws * args.steps / ins_s
@@ -193,7 +195,8 @@ def zipf_main(args):
         "roofline": {"kernel": "bulk_insert_reduce (whole call)", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_unit": "8 per draw + 162 per distinct id"},
-        "checks": {"exact_multiplicities": exact, "distinct_ids_drawn": distinct,
+        "reduce_path": "counting" if kern["reduce_finish"][1] else "sort",
+        "checks": {"exact_multiplicities": exact, "keys_left_sorted": keys_sorted, "distinct_ids_drawn": distinct,
                    "top_id_count": int(mult[0].item()), "status": f.status()},
     }
     f.close()

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgqf.so")
-SOURCES = ["gqf_api.hip", "gqf_sort.hip", "gqf_rebuild.hip", "gqf_query.hip", "gqf_image.hip", "gqf_point.hip", "gqf_group.hip", "gqf_enum.hip"]
+SOURCES = ["gqf_api.hip", "gqf_sort.hip", "gqf_rebuild.hip", "gqf_query.hip", "gqf_image.hip", "gqf_point.hip", "gqf_group.hip", "gqf_enum.hip",
+           "gqf_agg.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GQF_ARCH", "gfx950")
 

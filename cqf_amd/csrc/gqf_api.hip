@@ -54,6 +54,13 @@ struct Workspace {
   uint8_t* wide_flags = nullptr;     // nregions, kept zero by the kernels
   uint32_t* wide_count = nullptr;
   unsigned long long* misses = nullptr;
+  // counting reduce (allocated on first use): hash table, counts by
+  // compaction index, scan tile sums, control words
+  uint64_t* agg_keys = nullptr;
+  uint64_t* agg_cnt = nullptr;
+  uint64_t* agg_by_idx = nullptr;
+  uint64_t* agg_tiles = nullptr;
+  uint32_t* agg_ctrl = nullptr;
 };
 
 struct Filter {
@@ -137,6 +144,62 @@ void ensure_workspace(Filter* f, uint64_t n) {
   w.cap = cap;
 }
 
+// Counting reduce of a duplicate-heavy batch of hashed keys: the hashes are
+// counted in a table and only the distinct ones sorted (gqf_agg.hip), which
+// leaves `keys` and the (hash, count) pairs exactly as the sort and the
+// run-length reduce would.  Batches of at least 2^22 keys whose sample of
+// 4096 holds at most 2048 distinct hashes try it; a batch with more distinct
+// hashes than the table holds (3/4 of 2^22) falls back to the sort.  Reads
+// two control words back, so it synchronises the stream twice (the
+// reference's bulk_insert_reduce synchronises as well, src/gqf.cu:3247-3250,
+// and reads reduce_by_key's end back, :3273).  Returns false when not taken
+// (nothing in `keys` or the table changed); else *m_out = distinct hashes.
+// A stream under graph capture keeps the sort path (no host reads there).
+constexpr uint64_t kAggMinKeys = 1ULL << 22;
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+bool reduce_by_counting(Filter* f, uint64_t n, uint64_t* keys, int bits, hipStream_t s, uint64_t* m_out) {
+  Workspace& w = f->ws;
+  const int width = gqf_sort_width(bits, 0);
+  if (n < kAggMinKeys || bits > 40 || width + GQF_AGG_LOG_CAP > 64 || stream_capturing(s)) return false;
+  const uint64_t cap = 1ULL << GQF_AGG_LOG_CAP;
+  {
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (!w.agg_keys) {
+      dev_alloc(&w.agg_keys, cap);
+      dev_alloc(&w.agg_cnt, cap);
+      dev_alloc(&w.agg_by_idx, cap);
+      dev_alloc(&w.agg_tiles, 4096);
+      dev_alloc(&w.agg_ctrl, 4);
+      if (!w.agg_keys || !w.agg_cnt || !w.agg_by_idx || !w.agg_tiles || !w.agg_ctrl) return false;
+      gqf_launch_agg_clear(w.agg_keys, w.agg_cnt, cap, s);
+    }
+  }
+  const uint64_t mask = gqf_mask(f->host.key_bits);
+  const int vb = (int)f->host.value_bits;
+  uint32_t ctrl[4] = {0, 0, 0, 0};
+  gqf_launch_agg_sample(keys, n, mask, vb, w.agg_ctrl, s);
+  GQF_CHECK(hipMemcpyAsync(ctrl, w.agg_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
+  GQF_CHECK(hipStreamSynchronize(s));
+  if (ctrl[0] > GQF_AGG_SAMPLE / 2) return false;
+  gqf_launch_agg_tiles(keys, n, mask, vb, w.agg_keys, w.agg_cnt, GQF_AGG_LOG_CAP, w.agg_ctrl, s);
+  GQF_CHECK(hipMemcpyAsync(ctrl, w.agg_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
+  GQF_CHECK(hipStreamSynchronize(s));
+  const uint64_t m = ctrl[1];
+  if (ctrl[2] != 0 || m > cap / 4 * 3 || m == 0) {
+    gqf_launch_agg_clear(w.agg_keys, w.agg_cnt, cap, s);
+    return false;
+  }
+  gqf_launch_agg_finish(keys, n, m, width, w.agg_keys, w.agg_cnt, cap, w.agg_ctrl, w.upos, w.agg_by_idx, w.tmp,
+                        w.hist, w.totals, w.agg_tiles, w.tile_heads, w.ukeys, w.ucounts, w.upos, s);
+  *m_out = m;
+  return true;
+}
+
 void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool reduce, hipStream_t s) {
   if (n == 0) return;  // the reference's grid size underflows here
   Filter* f = lookup(qf);
@@ -146,6 +209,15 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
   const int vb = (int)f->host.value_bits;
   const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
+  uint64_t m = 0;
+  if (reduce && hash_it && reduce_by_counting(f, n, keys, bits, s, &m)) {
+    gqf_launch_region_bounds(f->dev, f->host.nregions, w.ukeys, m, nullptr, w.region_start, w.region_list,
+                             w.list_count, ~0ULL, s);
+    gqf_launch_rebuild(f->dev, f->host.nregions, w.ukeys, w.ucounts, m, w.region_start, w.region_list, w.list_count,
+                       mode, w.wide_flags, w.wide_count, s);
+    GQF_CHECK(hipGetLastError());
+    return;
+  }
   if (hash_it && gqf_sort_two_level_fits(n, bits))
     gqf_launch_sort_two_level(keys, w.tmp, n, gqf_mask(f->host.key_bits), vb, bits, w.hist, w.totals,
                               w.slow_buckets, s);
@@ -316,6 +388,11 @@ void free_all_workspace(Workspace& w) {
   dev_free(w.wide_flags);
   dev_free(w.wide_count);
   dev_free(w.misses);
+  dev_free(w.agg_keys);
+  dev_free(w.agg_cnt);
+  dev_free(w.agg_by_idx);
+  dev_free(w.agg_tiles);
+  dev_free(w.agg_ctrl);
 }
 
 }  // namespace

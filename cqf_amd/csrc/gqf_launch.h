@@ -17,11 +17,36 @@ void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uin
 // gqf_sort_two_level_fits: one stable pass on the top hash bits into
 // buckets of about 12K keys, then an LDS sort per bucket; not stable on
 // equal hashes (plain keys only: equal hashes are indistinguishable)
+// the low bits gqf_launch_sort orders for these arguments (passes x digit,
+// at least `bits`): bits above them ride along in their stable order
+int gqf_sort_width(int bits, int digit_bits);
 bool gqf_sort_two_level_fits(uint64_t n, int bits);
 // slow: GQF_SORT_MAX_BUCKETS flags, zero between calls (the kernels leave them so)
 #define GQF_SORT_MAX_BUCKETS 1024
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
                                uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s);
+
+// Counting reduce of duplicate-heavy batches (gqf_agg.hip).  ctrl: 4 words.
+// sample: ctrl[0] = distinct hashes among 4096 keys spread over the batch.
+#define GQF_AGG_SAMPLE 4096
+#define GQF_AGG_LOG_CAP 22  // global table of 2^22 (hash, count) entries
+void gqf_launch_agg_sample(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, uint32_t* ctrl,
+                           hipStream_t s);
+// counts every key's slot hash into the table (gkeys / gcnt, 2^log_cap
+// entries, empty on entry): ctrl[1] = distinct hashes, ctrl[2] != 0 if the
+// table ran out of room (then clear it with gqf_launch_agg_clear)
+void gqf_launch_agg_tiles(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, uint64_t* gkeys,
+                          uint64_t* gcnt, int log_cap, uint32_t* ctrl, hipStream_t s);
+void gqf_launch_agg_clear(uint64_t* gkeys, uint64_t* gcnt, uint64_t cap, hipStream_t s);
+// the table's m entries -> ascending ukeys / ucounts (m each), upos (m + 1
+// prefix sums) and keys = each hash count times in ascending order; empties
+// the table.  width = gqf_sort_width(hash bits, 0), width + log_cap <= 64;
+// comp: m words; cnt: 2^log_cap words; tmp / hist / totals: a sort of m
+// keys; tile_sums: 4096 words; starts: n / 4096 + 2 words.
+void gqf_launch_agg_finish(uint64_t* keys, uint64_t n, uint64_t m, int width, uint64_t* gkeys, uint64_t* gcnt,
+                           uint64_t cap, uint32_t* ctrl, uint64_t* comp, uint64_t* cnt, uint64_t* tmp, uint32_t* hist,
+                           uint32_t* totals, uint64_t* tile_sums, uint32_t* starts, uint64_t* ukeys,
+                           uint64_t* ucounts, uint64_t* upos, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
 // tile_heads: ceil(n/4096) words; upos: n words.

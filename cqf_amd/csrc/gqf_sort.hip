@@ -624,23 +624,36 @@ uint64_t gqf_sort_hist_words(uint64_t n) {
   return (uint64_t)kMaxRadix * (t > kScanSmallTiles ? t : kScanSmallTiles);
 }
 
+namespace {
+// radix digit and number of passes for `bits` key bits; an even number of
+// passes leaves the result in `keys`
+void sort_shape(int bits, int digit_bits, int* passes, int* digit) {
+  if (digit_bits >= 6 && digit_bits <= 10) {
+    *digit = digit_bits;
+    *passes = (bits + digit_bits - 1) / digit_bits;
+    *passes += *passes & 1;
+  } else if (bits <= 40) {
+    *passes = 4;
+    *digit = (bits + 3) / 4;
+    if (*digit < 6) *digit = 6;
+  } else {
+    *passes = 8;
+    *digit = 8;
+  }
+}
+}  // namespace
+
+int gqf_sort_width(int bits, int digit_bits) {
+  int passes, digit;
+  sort_shape(bits, digit_bits, &passes, &digit);
+  return passes * digit;
+}
+
 void gqf_launch_sort(uint64_t* keys, uint64_t* tmp, uint64_t n, int hash_it, uint64_t mask, int vshift, int bits,
                      int digit_bits, uint32_t* hist, uint32_t* totals, hipStream_t s) {
   if (n == 0) return;
-  // an even number of passes leaves the result in `keys`
   int passes, digit;
-  if (digit_bits >= 6 && digit_bits <= 10) {
-    digit = digit_bits;
-    passes = (bits + digit - 1) / digit;
-    passes += passes & 1;
-  } else if (bits <= 40) {
-    passes = 4;
-    digit = (bits + 3) / 4;
-    if (digit < 6) digit = 6;
-  } else {
-    passes = 8;
-    digit = 8;
-  }
+  sort_shape(bits, digit_bits, &passes, &digit);
   // small inputs: 1024-key tiles (more workgroups, shorter tiles)
   const bool small = n <= kSmallN;
   const uint32_t tile = small ? kThreads * kItemsSmall : kTile;
