@@ -257,9 +257,126 @@ gqf_sort_downsweep_small(const uint64_t* __restrict__ in, uint64_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Two-level sort of hashed keys (batches of 2^20..2^24): one stable pass of
-// the machinery above on the top `msd` hash bits scatters the keys into
-// 2^msd buckets; then one workgroup per bucket sorts the bucket's remaining
+// Top-digit split of hashed keys for the two-level sort.  Only the bucket
+// of each key matters here: the bucket sort then orders every bucket in
+// full, and keys with equal hashes are equal values, so the split need not
+// be stable.  That lets it use big tiles (kMRound keys per round, up to 8
+// rounds per workgroup: a 2 MB [digit][tile] histogram at 2^24 keys instead
+// of 16.8 MB of scattered counts) and rank keys within a round by LDS
+// atomics; each round's keys leave the workgroup grouped by digit, about
+// 64 bytes per digit at 1024 digits.
+
+namespace {
+constexpr int kMThreads = 1024;
+constexpr int kMWaves = kMThreads / 64;
+constexpr int kMItems = 8;
+constexpr uint32_t kMRound = kMThreads * kMItems;  // 8192 keys, 64 KB staged
+constexpr int kMRankShift = 48;                    // ranks < kMRound ride above the hash bits
+static_assert(kMRound <= (1u << (64 - kMRankShift)), "rank fits above the hash");
+static_assert(kMaxRadix <= kMThreads, "one digit per thread");
+}  // namespace
+
+// per-tile digit histogram of the hashed keys -> hist[d * ntiles + tile]
+extern "C" __global__ void __launch_bounds__(kMThreads)
+gqf_msd_count(const uint64_t* __restrict__ in, uint64_t n, uint64_t tile_keys, int shift, int bits, uint64_t mask,
+              int vshift, uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t h[kMaxRadix];
+  const int radix = 1 << bits;
+  const int t = threadIdx.x;
+  if (t < radix) h[t] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * tile_keys;
+  const uint64_t end = min(n, base + tile_keys);
+  for (uint64_t r0 = base; r0 < end; r0 += kMRound) {
+    uint64_t key[kMItems];
+#pragma unroll
+    for (int k = 0; k < kMItems; k++) {
+      const uint64_t i = r0 + (uint64_t)(k * kMThreads + t);
+      key[k] = i < end ? in[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kMItems; k++) {
+      const uint64_t i = r0 + (uint64_t)(k * kMThreads + t);
+      if (i < end) atomicAdd(&h[(uint32_t)((gqf_hash64(key[k], mask) << vshift) >> shift) & (radix - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  if (t < radix) hist[(uint64_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// scatter of one tile into the digit buckets (hist scanned per digit row,
+// bin_base the exclusive scan of the digit totals)
+extern "C" __global__ void __launch_bounds__(kMThreads, 8)
+gqf_msd_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, uint64_t tile_keys,
+                int shift, int bits, uint64_t mask, int vshift, const uint32_t* __restrict__ hist,
+                const uint32_t* __restrict__ bin_base, uint32_t ntiles) {
+  __shared__ uint64_t stage[kMRound];
+  __shared__ uint32_t cnt[kMaxRadix];     // keys of the round per digit
+  __shared__ uint32_t lstart[kMaxRadix];  // their first index in stage
+  __shared__ uint32_t glob[kMaxRadix];    // next output index of each digit
+  __shared__ uint32_t wsum[kMWaves];
+  const int radix = 1 << bits;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (t < radix) glob[t] = bin_base[t] + hist[(uint64_t)t * ntiles + blockIdx.x];
+  const uint64_t base = (uint64_t)blockIdx.x * tile_keys;
+  const uint64_t end = min(n, base + tile_keys);
+  for (uint64_t r0 = base; r0 < end; r0 += kMRound) {
+    // 32-bit offsets from the round's (uniform) base: one VGPR per address
+    const uint64_t* __restrict__ src = in + r0;
+    const uint32_t m = (uint32_t)min((uint64_t)kMRound, end - r0);
+    if (t < radix) cnt[t] = 0;
+    __syncthreads();
+    // each key's rank within its digit rides in bits 48.. of the key (the
+    // hashes here are below 2^42), which keeps the kernel within 64 VGPRs
+    uint64_t key[kMItems];
+#pragma unroll
+    for (int k = 0; k < kMItems; k++) {
+      const uint32_t i = (uint32_t)(k * kMThreads + t);
+      key[k] = i < m ? src[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kMItems; k++) {
+      const uint32_t i = (uint32_t)(k * kMThreads + t);
+      const uint64_t h = gqf_hash64(key[k], mask) << vshift;
+      const uint32_t rank = i < m ? atomicAdd(&cnt[(uint32_t)(h >> shift) & (radix - 1)], 1u) : 0u;
+      key[k] = h | ((uint64_t)rank << kMRankShift);
+    }
+    __syncthreads();
+    // exclusive scan of the digit counts, one digit per thread
+    const uint32_t c = t < radix ? cnt[t] : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t x = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += x;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    if (t < radix) lstart[t] = before + incl - c;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMItems; k++) {
+      const uint32_t i = (uint32_t)(k * kMThreads + t);
+      const uint64_t h = key[k] & ((1ULL << kMRankShift) - 1);
+      if (i < m) stage[lstart[(uint32_t)(h >> shift) & (radix - 1)] + (uint32_t)(key[k] >> kMRankShift)] = h;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < m; i += kMThreads) {
+      const uint64_t v = stage[i];
+      const uint32_t d = (uint32_t)(v >> shift) & (radix - 1);
+      out[(uint64_t)glob[d] + (i - lstart[d])] = v;
+    }
+    __syncthreads();
+    if (t < radix) glob[t] += c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-level sort of hashed keys (batches of 2^20..2^24): the top-digit
+// split above (gqf_msd_count / gqf_msd_scatter) puts the keys into 2^msd
+// buckets by their top `msd` hash bits; then one workgroup per bucket sorts the bucket's remaining
 // low bits (at most 32) in LDS -- up to kBucketCap keys as 32-bit values in
 // two 68 KB buffers of gfx950's 160 KB -- with stable 8-bit LSD passes whose
 // ranks come from wave ballots over peers of equal digit, as in the
@@ -693,15 +810,21 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
   int msd = 4;
   while (msd < 10 && (n >> msd) > 12288) msd++;
   if (bits - msd > 32) msd = bits - 32;
-  const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
   const int radix = 1 << msd;
+  // about 512 tiles of 1..8 rounds: 32K keys per tile at 2^24 keys.  The
+  // histogram (radix x ntiles <= 1024 x n / 8192 words) fits the LSD
+  // sort's (gqf_sort_hist_words)
+  uint64_t rounds = n / ((uint64_t)kMRound * 512);
+  rounds = rounds < 1 ? 1 : (rounds > 8 ? 8 : rounds);
+  const uint64_t tile_keys = rounds * kMRound;
+  const uint32_t ntiles = (uint32_t)((n + tile_keys - 1) / tile_keys);
   gqf_prof_begin("sort_pass", s);
-  hipLaunchKernelGGL(gqf_sort_upsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)keys, n, bits - msd, msd,
-                     1, mask, vshift, hist, ntiles);
+  hipLaunchKernelGGL(gqf_msd_count, dim3(ntiles), dim3(kMThreads), 0, s, (const uint64_t*)keys, n, tile_keys,
+                     bits - msd, msd, mask, vshift, hist, ntiles);
   hipLaunchKernelGGL(gqf_sort_scan_rows, dim3(radix), dim3(kThreads), 0, s, hist, ntiles, totals);
   hipLaunchKernelGGL(gqf_sort_scan_totals, dim3(1), dim3(kThreads), 0, s, totals, radix);
-  hipLaunchKernelGGL(gqf_sort_downsweep, dim3(ntiles), dim3(kThreads), 0, s, (const uint64_t*)keys, tmp, n, bits - msd,
-                     msd, 1, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
+  hipLaunchKernelGGL(gqf_msd_scatter, dim3(ntiles), dim3(kMThreads), 0, s, (const uint64_t*)keys, tmp, n, tile_keys,
+                     bits - msd, msd, mask, vshift, (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
   gqf_prof_end("sort_pass", s);
   gqf_prof_begin("sort_buckets", s);
   hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, (const uint64_t*)tmp, keys, n, bits - msd,
