@@ -399,22 +399,6 @@ struct BucketSmem {
   uint32_t run[256];                  // next free position of each digit
 };
 
-// Ascending bitonic sort of one value per lane across the wave.
-__device__ __forceinline__ uint32_t wave_bitonic_sort(uint32_t v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t p = (uint32_t)__shfl_xor((int)v, j, 64);
-      const bool up = (lane & k) == 0;
-      const bool lower = (lane & j) == 0;
-      v = (lower == up) ? (v < p ? v : p) : (v > p ? v : p);
-    }
-  }
-  return v;
-}
-
 // Stable scatter of src[0, m) into dst by digit (v >> sh) & 255, given the
 // digit totals; every thread of the workgroup calls it.
 template <typename V>
@@ -487,20 +471,25 @@ __device__ __forceinline__ void bucket_pass(const V* src, V* dst, uint32_t m, in
 }  // namespace
 
 // Common case, two workgroups per CU: the bucket's keys stay in registers
-// (17 per thread) while they are counted by sub-bucket -- the top (up to) 9
-// of the low bits, ~32 keys each for uniform hashes -- then go to one LDS
-// buffer in sub-bucket order (any order within: they are split again), and
-// each wave sorts whole sub-buckets of at most 64 keys with a bitonic
-// network across its lanes.  A bucket above kBucketCap keys, or with a
-// sub-bucket above 64 (repeated keys), is flagged for gqf_sort_buckets_slow.
+// (17 per thread) while they are counted by sub-bucket -- the top (up to) 11
+// of the low bits, ~8 keys each for uniform hashes -- then go to one LDS
+// buffer in sub-bucket order (any order within), and each key's place in
+// its sub-bucket is the number of keys there below it (ties by buffer
+// index), so every key is written straight to its sorted position.  A
+// bucket above kBucketCap keys, or with a sub-bucket above kSubCap
+// (repeated keys), is flagged for gqf_sort_buckets_slow.
 constexpr int kBLoad = (int)((kBucketCap + kBThreads - 1) / kBThreads);  // 17
+constexpr int kSubBitsMax = 11;
+constexpr uint32_t kSubMax = 1u << kSubBitsMax;
+constexpr uint32_t kSubCap = 64;
+static_assert(kSubMax == 2 * kBThreads, "two sub-buckets per thread in the scan");
 
 extern "C" __global__ void __launch_bounds__(kBThreads)
 gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int low_bits,
                  const uint32_t* __restrict__ starts, uint32_t nbuckets, uint32_t* __restrict__ slow) {
   __shared__ uint32_t buf[kBucketCap];
-  __shared__ uint32_t sub_cnt[512];    // counts, then cursors
-  __shared__ uint32_t sub_start[513];
+  __shared__ uint32_t sub_end[kSubMax];  // counts, then cursors: after the scatter, each sub-bucket's end
+  __shared__ uint32_t wsum[kBWaves];
   __shared__ uint32_t oversize;
   const uint32_t b = blockIdx.x;
   const uint64_t s = starts[b];
@@ -513,10 +502,10 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
     return;
   }
   const uint64_t hi = (uint64_t)b << low_bits;  // the bucket's top bits
-  const int db = low_bits < 9 ? low_bits : 9;
+  const int db = low_bits < kSubBitsMax ? low_bits : kSubBitsMax;
   const int sb = low_bits - db;
   const uint32_t nsub = 1u << db;
-  for (uint32_t d = t; d < nsub; d += kBThreads) sub_cnt[d] = 0;
+  for (uint32_t d = t; d < nsub; d += kBThreads) sub_end[d] = 0;
   if (t == 0) oversize = 0;
   uint32_t v[kBLoad];
 #pragma unroll
@@ -527,36 +516,26 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kBLoad; k++)
-    if ((uint32_t)(k * kBThreads + t) < m) atomicAdd(&sub_cnt[v[k] >> sb], 1u);
+    if ((uint32_t)(k * kBThreads + t) < m) atomicAdd(&sub_end[v[k] >> sb], 1u);
   __syncthreads();
-  if (w == 0) {  // exclusive scan of the sub-bucket counts, 8 per lane
-    uint32_t c[8], sum = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const uint32_t d = 8 * lane + k;
-      c[k] = d < nsub ? sub_cnt[d] : 0u;
-      sum += c[k];
-    }
+  {  // exclusive scan of the counts, two sub-buckets per thread
+    const uint32_t d0 = 2 * (uint32_t)t;
+    const uint32_t c0 = d0 < nsub ? sub_end[d0] : 0u;
+    const uint32_t c1 = d0 + 1 < nsub ? sub_end[d0 + 1] : 0u;
+    const uint32_t sum = c0 + c1;
     uint32_t incl = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t x = __shfl_up(incl, d, 64);
       if (lane >= d) incl += x;
     }
-    uint32_t r = incl - sum;
-    bool big = false;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const uint32_t d = 8 * lane + k;
-      if (d < nsub) {
-        sub_start[d] = r;
-        sub_cnt[d] = r;
-        big |= c[k] > 64u;
-      }
-      r += c[k];
-    }
-    if (lane == 63) sub_start[nsub] = r;
-    if (__ballot(big)) oversize = 1;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int k = 0; k < w; k++) run += wsum[k];
+    if (d0 < nsub) sub_end[d0] = run;
+    if (d0 + 1 < nsub) sub_end[d0 + 1] = run + c0;
+    if (c0 > kSubCap || c1 > kSubCap) oversize = 1;
   }
   __syncthreads();
   if (oversize) {
@@ -565,17 +544,20 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
   }
 #pragma unroll
   for (int k = 0; k < kBLoad; k++)
-    if ((uint32_t)(k * kBThreads + t) < m) buf[atomicAdd(&sub_cnt[v[k] >> sb], 1u)] = v[k];
+    if ((uint32_t)(k * kBThreads + t) < m) buf[atomicAdd(&sub_end[v[k] >> sb], 1u)] = v[k];
   __syncthreads();
-  for (uint32_t d = w; d < nsub; d += kBThreads / 64) {
-    const uint32_t st = sub_start[d], cnt = sub_start[d + 1] - st;
-    if (cnt == 0) continue;
-    uint32_t x = (uint32_t)lane < cnt ? buf[st + lane] : 0xFFFFFFFFu;
-    if (cnt > 1) x = wave_bitonic_sort(x);
-    if ((uint32_t)lane < cnt) buf[st + lane] = x;
+  for (uint32_t i = t; i < m; i += kBThreads) {
+    const uint32_t x = buf[i];
+    const uint32_t d = x >> sb;
+    const uint32_t st = d ? sub_end[d - 1] : 0u;
+    const uint32_t en = sub_end[d];
+    uint32_t r = 0;
+    for (uint32_t j = st; j < en; j++) {
+      const uint32_t y = buf[j];
+      r += (y < x || (y == x && j < i)) ? 1u : 0u;
+    }
+    out[s + st + r] = hi | x;
   }
-  __syncthreads();
-  for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | buf[i];
 }
 
 __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out, uint64_t n, int low_bits,
