@@ -776,7 +776,8 @@ __device__ __forceinline__ void rebuild_phase_item(const QF* __restrict__ qfp, c
   if (lane_id() == 0) {
     if (res == kNeedWide) {
       wide_flags[r] = 1;
-      atomicAdd(wide_count, 1u);
+      const uint32_t k = atomicAdd(wide_count, 1u);
+      if (k < GQF_WIDE_LIST) wide_count[1 + k] = (uint32_t)r;
     } else if (res == kNoSpace) {
       atomicOr(qf.status, GQF_STATUS_NO_SPACE);
     }
@@ -807,7 +808,10 @@ gqf_rebuild_phase_counts(const QF* __restrict__ qfp, const uint64_t* __restrict_
 // Redoes flagged regions of one parity in ascending order with a wider
 // window: 128 blocks, then 256 for a push that outruns that too (a region
 // that needs a wider window has not written anything).  One workgroup;
-// exits at once when nothing was flagged.
+// exits at once when nothing was flagged.  Up to GQF_WIDE_LIST flagged
+// regions come from the phase's list, ordered by rank; more than that, from
+// a scan of every region's flag (683 dependent rounds at 2^28 slots, about
+// 40 us whatever the number flagged).
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                  const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
@@ -816,28 +820,43 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   __shared__ Smem<kWinMid> smm;
   __shared__ Smem<kWinWide> sm;
   __shared__ uint32_t icnt[kItemCap];
-  if (*(volatile uint32_t*)wide_count == 0) return;
+  const uint32_t nflag = *(volatile uint32_t*)wide_count;
+  if (nflag == 0) return;
   const QF qf = *qfp;
   const int lane = lane_id();
-  for (uint64_t base = (uint64_t)parity; base < qf.nregions; base += 128) {
-    const uint64_t r = base + 2ULL * lane;
-    const bool f = r < qf.nregions && wide_flags[r];
-    uint64_t m = __ballot(f);
-    while (m) {
-      const int l = __builtin_ctzll(m);
-      m &= m - 1;
-      const uint64_t rr = base + 2ULL * l;
-      const uint64_t s = region_start[rr], e = region_start[rr + 1];
-      int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt);
-      __syncthreads();
-      if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
-      __syncthreads();
-      if (lane == 0) {
-        wide_flags[rr] = 0;
-        if (res == kNeedWide) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
-        else if (res == kNoSpace) atomicOr(qf.status, GQF_STATUS_NO_SPACE);
+  auto redo = [&](uint64_t rr) {
+    const uint64_t s = region_start[rr], e = region_start[rr + 1];
+    int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt);
+    __syncthreads();
+    if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
+    __syncthreads();
+    if (lane == 0) {
+      wide_flags[rr] = 0;
+      if (res == kNeedWide) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+      else if (res == kNoSpace) atomicOr(qf.status, GQF_STATUS_NO_SPACE);
+    }
+    __syncthreads();
+  };
+  if (nflag <= GQF_WIDE_LIST) {
+    // distinct regions: each one's rank among them is its place in the order
+    const uint32_t mine = (uint32_t)lane < nflag ? wide_count[1 + lane] : 0xFFFFFFFFu;
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < nflag; j++) rank += (uint32_t)__shfl((int)mine, (int)j, 64) < mine ? 1u : 0u;
+    for (uint32_t k = 0; k < nflag; k++) {
+      const uint64_t bm = __ballot((uint32_t)lane < nflag && rank == k);
+      if (!bm) continue;
+      redo((uint64_t)(uint32_t)__shfl((int)mine, __builtin_ctzll(bm), 64));
+    }
+  } else {
+    for (uint64_t base = (uint64_t)parity; base < qf.nregions; base += 128) {
+      const uint64_t r = base + 2ULL * lane;
+      const bool f = r < qf.nregions && wide_flags[r];
+      uint64_t m = __ballot(f);
+      while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        redo(base + 2ULL * l);
       }
-      __syncthreads();
     }
   }
   if (lane == 0) *wide_count = 0;

@@ -219,6 +219,35 @@ def test_tuning_does_not_change_results(gpu, oracle, region_blocks, digit):
         o.close()
 
 
+@pytest.mark.parametrize("nbits", [16, 22])
+def test_wide_window_fallback_at_high_load(gpu, oracle, nbits):
+    """56-block regions leave an 8-block push window, so at 95% load many
+    regions outrun it and are redone by the wide pass: from its list of
+    flagged regions at 2^16 slots (19 regions), and likely from the scan
+    of every region's flag at 2^22 (1,171 regions).  Inserts and deletes
+    must still give the oracle's image."""
+    from cqf_amd.gqf import to_device_u64, to_numpy_u64
+    _say(f"case wide fallback q={nbits} R=56 load 0.95")
+    keys = oracle.splitmix64(77, int(0.95 * (1 << nbits)))
+    g, o = _gpu_and_oracle(oracle, nbits)
+    try:
+        g.set_tuning(56, 0)
+        for part in np.array_split(keys, 4):
+            g.bulk_insert(to_device_u64(part))
+            o.bulk_insert(part.copy())
+            _assert_images_equal(g.export_image(), o.image(), "insert batch")
+        half = keys[1::2]
+        g.bulk_delete(to_device_u64(half))
+        o.bulk_delete(half.copy())
+        _assert_images_equal(g.export_image(), o.image(), "delete half")
+        got = to_numpy_u64(g.bulk_get(to_device_u64(keys)))
+        assert np.array_equal(got, o.bulk_get(keys))
+        assert g.status() == 0
+    finally:
+        g.close()
+        o.close()
+
+
 @pytest.mark.parametrize("nbits", [20, 22])
 def test_sparse_batches_after_dense_fill(gpu, oracle, nbits):
     """Batches of at most one key per point-lock region take the point path
