@@ -10,8 +10,8 @@
 //   1. gqf_agg_sample hashes 4096 keys spread over the batch and counts
 //      the distinct hashes among them (the host skips this path when more
 //      than half are distinct);
-//   2. gqf_agg_tiles: each workgroup counts the hashes of 16384 keys in an
-//      LDS hash table, then adds its (hash, count) entries to one global
+//   2. gqf_agg_tiles: each workgroup counts the hashes of 131072 keys in an
+//      8192-entry LDS hash table, then adds its (hash, count) entries to one global
 //      open-addressing table (a key its LDS table has no room for goes
 //      straight to the global table).  Before its LDS inserts, a wave
 //      merges the lanes holding its most frequent hashes by ballot, so a
@@ -34,11 +34,19 @@ constexpr uint64_t kEmpty = ~0ULL - 1;  // no entry: hashes are below 2^48 (and 
 constexpr int kThreads = 256;
 constexpr int kSample = GQF_AGG_SAMPLE;
 constexpr int kSampleCap = 8192;
-constexpr int kTileKeys = 16384;         // keys per workgroup of gqf_agg_tiles
+constexpr int kTileThreads = 1024;       // threads per workgroup of gqf_agg_tiles
+constexpr int kTileKeys = 131072;        // keys per workgroup of gqf_agg_tiles
+// (2^28 Zipf(1.5) draws, ms: 16K keys / 2048 entries / 256 threads 4.19;
+// 64K / 8192 / 1024 4.13; 128K / 8192 / 1024 3.25; 256K / 8192 / 1024 4.60;
+// 128K / 16384 / 1024 4.35 -- one workgroup per CU)
 constexpr int kWaveRounds = 2;           // hashes a wave merges by ballot before its LDS inserts
                                          // (1: 4.18 ms, 2: 4.20, 4: 4.49, 6: 4.78 for 2^28 Zipf draws)
 constexpr int kLoadsPerRound = 8;        // keys each thread loads before inserting them
-constexpr int kLocalCap = 2048;          // LDS table entries
+constexpr int kLocalLog = 13;
+constexpr int kLocalCap = 1 << kLocalLog;  // LDS table entries
+constexpr int kCountBits = 24;             // count bits of an LDS entry
+constexpr unsigned long long kLocalEmpty = ~0ULL;
+static_assert(kTileKeys < (1 << kCountBits), "a workgroup's counts fit an LDS entry");
 constexpr int kLocalProbe = 32;
 constexpr int kGlobalProbe = 128;
 constexpr int kScanItems = 16;
@@ -127,33 +135,34 @@ gqf_agg_sample(const uint64_t* __restrict__ keys, uint64_t n, uint64_t mask, int
   }
 }
 
-extern "C" __global__ void __launch_bounds__(kThreads)
+extern "C" __global__ void __launch_bounds__(kTileThreads)
 gqf_agg_tiles(const uint64_t* __restrict__ keys, uint64_t n, uint64_t mask, int vshift,
               unsigned long long* __restrict__ gkeys, unsigned long long* __restrict__ gcnt, int log_cap,
               uint32_t* __restrict__ ctrl) {
-  // Measured on 2^28 Zipf(1.5) draws: 1.7 ms of the kernel's 4.2 ms are the
-  // loads and the LDS counting, the rest the adds to the global table
-  // (about 16M, one per distinct hash per workgroup).  Eight copies of the
+  // Each distinct hash of a workgroup's tile costs one add to the global
+  // table, and at Zipf(1.5) a tile of T draws holds about T^(2/3) distinct
+  // hashes, so bigger tiles make fewer adds (T^(-1/3) per draw).  With
+  // 16K-draw tiles (about 16M adds) 1.7 ms of the kernel's 4.2 ms were the
+  // loads and the LDS counting, the rest the adds.  Eight copies of the
   // counts, one per XCD's workgroups, left the time as it was: the adds are
   // bound by device-scope atomic throughput, not by same-address chains.
+  // LDS entry: (hash << kCountBits) | count, kLocalEmpty when free (hashes
+  // are below 2^40 here and a workgroup counts fewer than 2^24 keys), so an
+  // 8192-entry table is 64 KB and two workgroups fit a CU
   __shared__ unsigned long long tk[kLocalCap];
-  __shared__ uint32_t tc[kLocalCap];
   __shared__ uint32_t fill;
-  for (int i = threadIdx.x; i < kLocalCap; i += kThreads) {
-    tk[i] = kEmpty;
-    tc[i] = 0;
-  }
+  for (int i = threadIdx.x; i < kLocalCap; i += kTileThreads) tk[i] = kLocalEmpty;
   if (threadIdx.x == 0) fill = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   // once the table is half full a hash not yet in it goes to the global
   // table directly (short probes whatever the batch)
   for (uint64_t base = (uint64_t)blockIdx.x * kTileKeys; base < n; base += (uint64_t)gridDim.x * kTileKeys) {
-    for (int r = 0; r < kTileKeys; r += kThreads * kLoadsPerRound) {
+    for (int r = 0; r < kTileKeys; r += kTileThreads * kLoadsPerRound) {
       uint64_t h[kLoadsPerRound];
 #pragma unroll
       for (int k = 0; k < kLoadsPerRound; k++) {
-        const uint64_t i = base + (uint64_t)(r + k * kThreads + (int)threadIdx.x);
+        const uint64_t i = base + (uint64_t)(r + k * kTileThreads + (int)threadIdx.x);
         h[k] = i < n ? gqf_hash64(keys[i], mask) << vshift : kEmpty;
       }
 #pragma unroll
@@ -178,17 +187,21 @@ gqf_agg_tiles(const uint64_t* __restrict__ keys, uint64_t n, uint64_t mask, int 
           }
         }
         if (!pend) continue;
-        uint32_t s = slot_of(hk, 11);
+        uint32_t s = slot_of(hk, kLocalLog);
         bool done = false;
+        const unsigned long long tag = (unsigned long long)hk << kCountBits;
         for (int p = 0; p < kLocalProbe; p++) {
           unsigned long long cur = *(volatile unsigned long long*)&tk[s];
-          if (cur == kEmpty) {
+          if (cur == kLocalEmpty) {
             if (*(volatile uint32_t*)&fill >= kLocalCap / 2) break;
-            cur = atomicCAS(&tk[s], kEmpty, hk);
-            if (cur == kEmpty) atomicAdd(&fill, 1u);
+            cur = atomicCAS(&tk[s], kLocalEmpty, tag);
+            if (cur == kLocalEmpty) {
+              atomicAdd(&fill, 1u);
+              cur = tag;
+            }
           }
-          if (cur == kEmpty || cur == hk) {
-            atomicAdd(&tc[s], c);
+          if ((cur >> kCountBits) == hk) {
+            atomicAdd(&tk[s], (unsigned long long)c);
             done = true;
             break;
           }
@@ -199,9 +212,9 @@ gqf_agg_tiles(const uint64_t* __restrict__ keys, uint64_t n, uint64_t mask, int 
     }
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < kLocalCap; s += kThreads) {
-    const uint64_t h = tk[s];
-    if (h != kEmpty) global_add(gkeys, gcnt, log_cap, h, tc[s], ctrl);
+  for (int s = threadIdx.x; s < kLocalCap; s += kTileThreads) {
+    const uint64_t e = tk[s];
+    if (e != kLocalEmpty) global_add(gkeys, gcnt, log_cap, e >> kCountBits, e & ((1ULL << kCountBits) - 1), ctrl);
   }
 }
 
@@ -373,12 +386,13 @@ void gqf_launch_agg_sample(const uint64_t* keys, uint64_t n, uint64_t mask, int 
 
 void gqf_launch_agg_tiles(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, uint64_t* gkeys,
                           uint64_t* gcnt, int log_cap, uint32_t* ctrl, hipStream_t s) {
-  // one workgroup per tile.  Measured on 2^28 Zipf(1.5) draws: 4.2 ms;
-  // 1024 workgroups striding over the tiles took 6.7 ms (the hashes beyond
-  // their tables' first half cost one global add each)
+  // one workgroup per tile, two resident per CU (64 KB of LDS each).  With
+  // 16K-draw tiles, 1024 workgroups striding over the tiles took 6.7 ms
+  // against 4.2 ms for one per tile (the hashes beyond their tables' first
+  // half cost one global add each)
   const uint64_t g = (n + kTileKeys - 1) / kTileKeys;
   gqf_prof_begin("reduce_count", s);
-  hipLaunchKernelGGL(gqf_agg_tiles, dim3((uint32_t)g), dim3(kThreads), 0, s, keys, n, mask, vshift,
+  hipLaunchKernelGGL(gqf_agg_tiles, dim3((uint32_t)g), dim3(kTileThreads), 0, s, keys, n, mask, vshift,
                      (unsigned long long*)gkeys, (unsigned long long*)gcnt, log_cap, ctrl);
   gqf_prof_end("reduce_count", s);
 }
