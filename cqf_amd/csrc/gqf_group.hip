@@ -80,6 +80,14 @@ gqf_group_tag(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n, int is_h
   }
 }
 
+// the filter index above the hash bits of keys already hashed and sorted
+// per segment
+extern "C" __global__ void __launch_bounds__(256)
+gqf_group_tag_sorted(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] |= (uint64_t)seg_of(g.seg, g.nf, i) << g.tag_shift;
+}
+
 extern "C" __global__ void __launch_bounds__(256)
 gqf_group_untag(uint64_t* __restrict__ keys, uint64_t n, int tag_shift) {
   const uint64_t m = (1ULL << tag_shift) - 1;
@@ -118,6 +126,7 @@ struct Group {
   uint64_t* tmp = nullptr;
   uint32_t* hist = nullptr;
   uint32_t* totals = nullptr;
+  uint32_t* slow = nullptr;   // GQF_SORT_MAX_BUCKETS flags of the two-level sort, zero between calls
   uint64_t cap = 0;
   uint32_t per_cu = 0;  // workgroups per CU of the persistent rebuild (0: default)
   // graph cache
@@ -209,12 +218,39 @@ void ensure_sort_space(Group* g, uint64_t n) {
   dalloc(&g->hist, gqf_sort_hist_words(g->cap));
 }
 
+// Keys to be hashed whose every segment fits the two-level sort (at most
+// 2^24 keys, 18..42 hash bits) are sorted segment by segment, hashing on
+// load, and tagged afterwards: equal (filter, hash) values are equal keys,
+// so the unstable two-level sort gives the same array as one LSD sort of
+// the tagged keys (4 passes over 40 bits).  The 256-filter k-mer group's
+// insert call (7.3e8 keys) went from 147 ms to 104 ms.
+bool per_segment_sort(const Group* g, int is_hash) {
+  if (is_hash) return false;
+  for (uint32_t f = 0; f < g->nf; f++) {
+    const uint64_t m = g->h_tab[f + 1] - g->h_tab[f];
+    const int bits = (int)(g->h_descs[f].key_bits + g->h_descs[f].value_bits);
+    if (m > (1ULL << 24) || bits < 18 || bits > 42) return false;
+  }
+  return true;
+}
+
 void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t n, int is_hash, int mode,
                     hipStream_t s) {
-  hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
-  uint32_t fbits = 0;
-  while ((1ULL << fbits) < dev.nf) fbits++;
-  gqf_launch_sort(keys, g->tmp, n, 0, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
+  if (per_segment_sort(g, is_hash)) {
+    for (uint32_t f = 0; f < g->nf; f++) {
+      const uint64_t a = g->h_tab[f], m = g->h_tab[f + 1] - a;
+      if (m == 0) continue;
+      const QF& d = g->h_descs[f];
+      gqf_launch_sort_two_level(keys + a, g->tmp + a, m, gqf_mask(d.key_bits), (int)d.value_bits,
+                                (int)(d.key_bits + d.value_bits), g->hist, g->totals, g->slow, s);
+    }
+    hipLaunchKernelGGL(gqf_group_tag_sorted, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n);
+  } else {
+    hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
+    uint32_t fbits = 0;
+    while ((1ULL << fbits) < dev.nf) fbits++;
+    gqf_launch_sort(keys, g->tmp, n, 0, 0, 0, dev.tag_shift + (int)fbits, 0, g->hist, g->totals, s);
+  }
   hipLaunchKernelGGL(gqf_group_reset, dim3(1), dim3(64), 0, s, g->d_counters);
   gqf_launch_group_rebuild(dev, keys, mode, gqf_group_rebuild_grid(g->per_cu), s);
   hipLaunchKernelGGL(gqf_group_untag, dim3(flat_grid(n)), dim3(256), 0, s, keys, n, dev.tag_shift);
@@ -240,7 +276,9 @@ extern "C" int gqf_group_create(gqf_group** out, QF* const* filters, uint32_t nf
   dalloc(&g->d_tab, 4 * (size_t)(nfilters + 1));
   dalloc(&g->d_counters, 4);
   dalloc(&g->totals, 1024);
-  if (!g->d_descs || !g->d_tab || !g->d_counters || !g->totals) {
+  dalloc(&g->slow, GQF_SORT_MAX_BUCKETS);
+  if (g->slow) GCHECK(hipMemset(g->slow, 0, GQF_SORT_MAX_BUCKETS * sizeof(uint32_t)));
+  if (!g->d_descs || !g->d_tab || !g->d_counters || !g->totals || !g->slow) {
     delete g;
     return -3;
   }
@@ -275,6 +313,7 @@ extern "C" void gqf_group_destroy(gqf_group* h) {
   dfree(g->tmp);
   dfree(g->hist);
   dfree(g->totals);
+  dfree(g->slow);
   delete g;
 }
 
