@@ -218,10 +218,18 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
     GQF_CHECK(hipGetLastError());
     return;
   }
-  if (hash_it && gqf_sort_two_level_fits(n, bits))
+  // the two-level sort also writes the region starts of a plain batch
+  // (reduce batches index their distinct keys instead)
+  const bool two_level = hash_it && gqf_sort_two_level_fits(n, bits);
+  if (two_level) {
+    gqf_region_args ra;
+    ra.start = w.region_start;
+    ra.step = f->host.region_blocks << 14;
+    ra.nregions = f->host.nregions;
+    ra.range = f->host.range;
     gqf_launch_sort_two_level(keys, w.tmp, n, gqf_mask(f->host.key_bits), vb, bits, w.hist, w.totals,
-                              w.slow_buckets, s);
-  else
+                              w.slow_buckets, s, reduce ? nullptr : &ra);
+  } else
     gqf_launch_sort(keys, w.tmp, n, hash_it ? 1 : (vb ? 2 : 0), gqf_mask(f->host.key_bits), vb, bits, f->sort_digit,
                     w.hist, w.totals, s);
   // A sparse insert batch -- at most one key per point-lock region and one
@@ -244,8 +252,11 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
     counts = w.ucounts;
     n_dev = w.m_dev;
   }
-  gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, w.region_list, w.list_count,
-                           ~0ULL, s);
+  if (two_level && !reduce)
+    gqf_launch_region_lists(f->dev, f->host.nregions, n, w.region_start, w.region_list, w.list_count, s);
+  else
+    gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, w.region_list,
+                             w.list_count, ~0ULL, s);
   gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, n, w.region_start, w.region_list, w.list_count, mode,
                      w.wide_flags, w.wide_count, s);
   GQF_CHECK(hipGetLastError());

@@ -23,8 +23,19 @@ int gqf_sort_width(int bits, int digit_bits);
 bool gqf_sort_two_level_fits(uint64_t n, int bits);
 // slow: GQF_SORT_MAX_BUCKETS flags, zero between calls (the kernels leave them so)
 #define GQF_SORT_MAX_BUCKETS 1024
+// Region starts from the bucket sort (optional): for every region r <
+// nregions whose bound r * step lies below range, start[r] = the first index
+// of the sorted keys at or above the bound (what gqf_region_bounds would
+// write); gqf_launch_region_lists then completes the table and the lists.
+struct gqf_region_args {
+  uint32_t* start = nullptr;
+  uint64_t step = 0;      // hash values per region: region_blocks << 14
+  uint64_t nregions = 0;
+  uint64_t range = 0;
+};
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
-                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s);
+                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s,
+                               const gqf_region_args* regions = nullptr);
 
 // Counting reduce of duplicate-heavy batches (gqf_agg.hip).  ctrl: 4 words.
 // sample: ctrl[0] = distinct hashes among 4096 keys spread over the batch.
@@ -59,6 +70,11 @@ void gqf_launch_rle(const uint64_t* keys, uint64_t n, uint32_t* tile_heads, uint
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
                               uint32_t* list_count, uint64_t hmask, hipStream_t s);
+// region_start entries that gqf_launch_sort_two_level leaves (bounds at or
+// past range, and r = nregions: n) and the non-empty region lists, as
+// gqf_launch_region_bounds makes them
+void gqf_launch_region_lists(const QF* qf_dev, uint64_t nregions, uint64_t n, uint32_t* region_start,
+                             uint32_t* region_list, uint32_t* list_count, hipStream_t s);
 
 // Sequenced batches: a sorted item is its slot hash with the request tag
 // above GQF_SEQ_SHIFT -- the request's sequence number in bits 48..62 and

@@ -867,12 +867,52 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
 // Also appends each non-empty region r < nregions to region_list (at
 // parity r & 1, list stride `list_stride`), counting in list_count[parity]
 // (zeroed by gqf_region_lists_reset just before).
-extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
-                                             uint64_t n_host, const uint64_t* __restrict__ n_dev,
-                                             uint32_t* __restrict__ region_start, uint32_t* __restrict__ region_list,
-                                             uint32_t* __restrict__ list_count, uint64_t list_stride, uint64_t hmask) {
+//
+// Each wave covers 63 regions plus the next wave's first: every lane runs
+// one binary search, and a region is non-empty when the next lane's start
+// is above its own (one chain of dependent loads per lane instead of two).
+constexpr int kBoundsPerWave = 63;
+constexpr int kBoundsThreads = 1024;
+
+// Appends the non-empty regions of the workgroup to the list of their
+// parity with ONE atomic per workgroup and parity: same-address atomics
+// serialise, and one per wave (2,800 at 2^28 slots) took about 30 us.
+__device__ __forceinline__ void append_region_lists(bool nonempty, uint64_t r, uint32_t* __restrict__ region_list,
+                                                    uint32_t* __restrict__ list_count, uint64_t list_stride) {
+  __shared__ uint32_t wcnt[2][kBoundsThreads / 64];
+  __shared__ uint32_t wbase[2];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  const uint64_t m0 = __ballot(nonempty && (r & 1) == 0);
+  const uint64_t m1 = __ballot(nonempty && (r & 1) == 1);
+  if (lane == 0) {
+    wcnt[0][w] = (uint32_t)__popcll(m0);
+    wcnt[1][w] = (uint32_t)__popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); k++) tot += wcnt[threadIdx.x][k];
+    wbase[threadIdx.x] = tot ? atomicAdd(&list_count[threadIdx.x], tot) : 0u;
+  }
+  __syncthreads();
+  if (nonempty) {
+    const uint32_t p = (uint32_t)(r & 1);
+    uint32_t base = wbase[p];
+    for (uint32_t k = 0; k < w; k++) base += wcnt[p][k];
+    region_list[p * list_stride + base + (uint32_t)__popcll((p ? m1 : m0) & lt)] = (uint32_t)r;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(kBoundsThreads)
+gqf_region_bounds(const QF* __restrict__ qfp, const uint64_t* __restrict__ items, uint64_t n_host,
+                  const uint64_t* __restrict__ n_dev, uint32_t* __restrict__ region_start,
+                  uint32_t* __restrict__ region_list, uint32_t* __restrict__ list_count, uint64_t list_stride,
+                  uint64_t hmask) {
   const QF qf = *qfp;
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t r = wave * kBoundsPerWave + lane;
   const bool valid = r <= qf.nregions;
   const uint64_t n = n_dev ? *n_dev : n_host;
   auto lower = [&](uint64_t bound) {
@@ -883,30 +923,41 @@ extern "C" __global__ void gqf_region_bounds(const QF* __restrict__ qfp, const u
     }
     return lo;
   };
-  bool nonempty = false;
+  uint64_t lo = 0;
   if (valid) {
-    const uint64_t lo = lower(r == qf.nregions ? qf.range : (r * qf.region_blocks) << 14);
-    region_start[r] = (uint32_t)lo;
-    if (r == qf.nregions) {
-      if (lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
+    lo = lower(r == qf.nregions ? qf.range : (r * qf.region_blocks) << 14);
+    region_start[r] = (uint32_t)lo;  // a wave's last lane repeats the next wave's first
+    if (r == qf.nregions && lo < n) atomicOr(qf.status, GQF_STATUS_BAD_HASH);
+  }
+  const uint64_t next = __shfl_down(lo, 1, 64);
+  const bool nonempty = valid && lane < (uint32_t)kBoundsPerWave && r < qf.nregions && next > lo;
+  append_region_lists(nonempty, r, region_list, list_count, list_stride);
+}
+
+// The region table after a two-level sort that wrote start[r] for every
+// region whose bound lies below range: the other entries (bounds at or past
+// range, and r = nregions) are n, and the non-empty regions are listed as
+// gqf_region_bounds lists them (same wave layout, no search).
+extern "C" __global__ void __launch_bounds__(kBoundsThreads)
+gqf_region_lists(const QF* __restrict__ qfp, uint64_t n, uint32_t* __restrict__ region_start,
+                 uint32_t* __restrict__ region_list, uint32_t* __restrict__ list_count, uint64_t list_stride) {
+  const QF qf = *qfp;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t r = wave * kBoundsPerWave + lane;
+  const bool valid = r <= qf.nregions;
+  uint64_t lo = 0;
+  if (valid) {
+    if (r == qf.nregions || ((r * qf.region_blocks) << 14) >= qf.range) {
+      lo = n;
+      region_start[r] = (uint32_t)n;
     } else {
-      nonempty = lower(r + 1 == qf.nregions ? qf.range : ((r + 1) * qf.region_blocks) << 14) > lo;
+      lo = region_start[r];
     }
   }
-  // append, one atomic per wave and parity (the lanes of a wave hold
-  // consecutive regions, so even lanes are even regions when the wave's
-  // first region is even)
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
-  for (uint32_t p = 0; p < 2; p++) {
-    const uint64_t m = __ballot(nonempty && (uint32_t)(r & 1) == p);
-    if (!m) continue;
-    uint32_t base = 0;
-    if (lane == (uint32_t)__builtin_ctzll(m)) base = atomicAdd(&list_count[p], (uint32_t)__popcll(m));
-    base = __shfl(base, __builtin_ctzll(m), 64);
-    if (nonempty && (uint32_t)(r & 1) == p)
-      region_list[p * list_stride + base + (uint32_t)__popcll(m & lt)] = (uint32_t)r;
-  }
+  const uint64_t next = __shfl_down(lo, 1, 64);
+  const bool nonempty = valid && lane < (uint32_t)kBoundsPerWave && r < qf.nregions && next > lo;
+  append_region_lists(nonempty, r, region_list, list_count, list_stride);
 }
 
 extern "C" __global__ void gqf_region_lists_reset(uint32_t* __restrict__ list_count) {
@@ -1062,11 +1113,24 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
                               uint32_t* list_count, uint64_t hmask, hipStream_t s) {
-  const uint32_t g = (uint32_t)((nregions + 1 + 255) / 256);
+  // waves of kBoundsPerWave regions (+1 overlap), kBoundsThreads / 64 per workgroup
+  const uint64_t waves = (nregions + 1 + kBoundsPerWave - 1) / kBoundsPerWave;
+  const uint32_t g = (uint32_t)((waves + kBoundsThreads / 64 - 1) / (kBoundsThreads / 64));
   gqf_prof_begin("region_bounds", s);
   hipLaunchKernelGGL(gqf_region_lists_reset, dim3(1), dim3(64), 0, s, list_count);
-  hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(256), 0, s, qf_dev, items, n, n_dev, region_start,
+  hipLaunchKernelGGL(gqf_region_bounds, dim3(g), dim3(kBoundsThreads), 0, s, qf_dev, items, n, n_dev, region_start,
                      region_list, list_count, (uint64_t)((nregions + 1) / 2 + 1), hmask);
+  gqf_prof_end("region_bounds", s);
+}
+
+void gqf_launch_region_lists(const QF* qf_dev, uint64_t nregions, uint64_t n, uint32_t* region_start,
+                             uint32_t* region_list, uint32_t* list_count, hipStream_t s) {
+  const uint64_t waves = (nregions + 1 + kBoundsPerWave - 1) / kBoundsPerWave;
+  const uint32_t g = (uint32_t)((waves + kBoundsThreads / 64 - 1) / (kBoundsThreads / 64));
+  gqf_prof_begin("region_bounds", s);
+  hipLaunchKernelGGL(gqf_region_lists_reset, dim3(1), dim3(64), 0, s, list_count);
+  hipLaunchKernelGGL(gqf_region_lists, dim3(g), dim3(kBoundsThreads), 0, s, qf_dev, n, region_start, region_list, list_count,
+                     (uint64_t)((nregions + 1) / 2 + 1));
   gqf_prof_end("region_bounds", s);
 }
 

@@ -470,6 +470,25 @@ __device__ __forceinline__ void bucket_pass(const V* src, V* dst, uint32_t m, in
 
 }  // namespace
 
+// start[r] of the regions whose bounds lie in bucket b's hash range
+// [b << low_bits, (b + 1) << low_bits): the bucket's first index s plus
+// below(bound - bucket base), the number of its keys whose low bits are
+// below that (a bound at or past range is left to gqf_region_lists)
+template <typename F>
+__device__ __forceinline__ void bucket_region_starts(const gqf_region_args& ra, uint32_t b, int low_bits, uint64_t s,
+                                                     F below) {
+  if (!ra.start) return;
+  const uint64_t lo = (uint64_t)b << low_bits;
+  const uint64_t hi = lo + (1ULL << low_bits);
+  const uint64_t r0 = (lo + ra.step - 1) / ra.step;
+  uint64_t r1 = (hi + ra.step - 1) / ra.step;
+  if (r1 > ra.nregions) r1 = ra.nregions;
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const uint64_t bound = r * ra.step;
+    if (bound < ra.range) ra.start[r] = (uint32_t)(s + below((uint32_t)(bound - lo)));
+  }
+}
+
 // Common case, two workgroups per CU: the bucket's keys stay in registers
 // (17 per thread) while they are counted by sub-bucket -- the top (up to) 11
 // of the low bits, ~8 keys each for uniform hashes -- then go to one LDS
@@ -486,7 +505,8 @@ static_assert(kSubMax == 2 * kBThreads, "two sub-buckets per thread in the scan"
 
 extern "C" __global__ void __launch_bounds__(kBThreads)
 gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int low_bits,
-                 const uint32_t* __restrict__ starts, uint32_t nbuckets, uint32_t* __restrict__ slow) {
+                 const uint32_t* __restrict__ starts, uint32_t nbuckets, uint32_t* __restrict__ slow,
+                 gqf_region_args ra) {
   __shared__ uint32_t buf[kBucketCap];
   __shared__ uint32_t sub_end[kSubMax];  // counts, then cursors: after the scatter, each sub-bucket's end
   __shared__ uint32_t wsum[kBWaves];
@@ -494,7 +514,10 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
   const uint32_t b = blockIdx.x;
   const uint64_t s = starts[b];
   const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
-  if (e <= s) return;
+  if (e <= s) {
+    bucket_region_starts(ra, b, low_bits, s, [](uint32_t) { return 0u; });
+    return;
+  }
   const uint32_t m = (uint32_t)(e - s);
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (m > kBucketCap) {
@@ -558,11 +581,19 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
     }
     out[s + st + r] = hi | x;
   }
+  bucket_region_starts(ra, b, low_bits, s, [&](uint32_t x) {
+    const uint32_t d = x >> sb;
+    const uint32_t st = d ? sub_end[d - 1] : 0u, en = sub_end[d];
+    uint32_t c = st;
+    for (uint32_t j = st; j < en; j++) c += buf[j] < x ? 1u : 0u;
+    return c;
+  });
 }
 
 __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out, uint64_t n, int low_bits,
                                                  const uint32_t* __restrict__ starts, uint32_t nbuckets,
-                                                 uint32_t* __restrict__ slow, uint32_t b, BucketSmem& sm) {
+                                                 uint32_t* __restrict__ slow, uint32_t b, BucketSmem& sm,
+                                                 const gqf_region_args& ra) {
   const uint64_t s = starts[b];
   const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
   const uint32_t m = (uint32_t)(e - s);
@@ -586,6 +617,15 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
       src ^= 1;
     }
     for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | sm.buf[src][i];
+    const uint32_t* sorted = sm.buf[src];
+    bucket_region_starts(ra, b, low_bits, s, [&](uint32_t x) {
+      uint32_t lo = 0, hi2 = m;
+      while (lo < hi2) {
+        const uint32_t mid = (lo + hi2) >> 1;
+        if (sorted[mid] < x) lo = mid + 1; else hi2 = mid;
+      }
+      return lo;
+    });
     return;
   }
   // too large for LDS: the same passes between the bucket's ranges of in and
@@ -606,6 +646,16 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
+  // the bucket's keys now ascend in o (their top bits are the bucket's)
+  const uint64_t base = (uint64_t)b << low_bits;
+  bucket_region_starts(ra, b, low_bits, s, [&](uint32_t x) {
+    uint32_t lo = 0, hi2 = m;
+    while (lo < hi2) {
+      const uint32_t mid = (lo + hi2) >> 1;
+      if (o[mid] - base < (uint64_t)x) lo = mid + 1; else hi2 = mid;
+    }
+    return lo;
+  });
 }
 
 // The flagged buckets (skips the others, and clears the flags it takes):
@@ -616,10 +666,10 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
 // one short launch.
 extern "C" __global__ void __launch_bounds__(kBThreads)
 gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
-                      uint32_t nbuckets, uint32_t* __restrict__ slow) {
+                      uint32_t nbuckets, uint32_t* __restrict__ slow, gqf_region_args ra) {
   __shared__ BucketSmem sm;
   for (uint32_t b = blockIdx.x; b < nbuckets; b += gridDim.x) {
-    if (slow[b]) sort_flagged_bucket(in, out, n, low_bits, starts, nbuckets, slow, b, sm);
+    if (slow[b]) sort_flagged_bucket(in, out, n, low_bits, starts, nbuckets, slow, b, sm, ra);
     __syncthreads();
   }
 }
@@ -787,7 +837,9 @@ bool gqf_sort_two_level_fits(uint64_t n, int bits) {
 }
 
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
-                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s) {
+                               uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s,
+                               const gqf_region_args* regions) {
+  const gqf_region_args ra = regions ? *regions : gqf_region_args{};
   // 2^msd buckets of about 12K keys (at most 1024), low bits <= 32
   int msd = 4;
   while (msd < 10 && (n >> msd) > 12288) msd++;
@@ -810,9 +862,9 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
   gqf_prof_end("sort_pass", s);
   gqf_prof_begin("sort_buckets", s);
   hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, (const uint64_t*)tmp, keys, n, bits - msd,
-                     (const uint32_t*)totals, (uint32_t)radix, slow);
+                     (const uint32_t*)totals, (uint32_t)radix, slow, ra);
   hipLaunchKernelGGL(gqf_sort_buckets_slow, dim3(radix < 256 ? radix : 256), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
-                     (const uint32_t*)totals, (uint32_t)radix, slow);
+                     (const uint32_t*)totals, (uint32_t)radix, slow, ra);
   gqf_prof_end("sort_buckets", s);
 }
 
