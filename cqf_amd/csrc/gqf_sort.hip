@@ -37,7 +37,15 @@ __device__ inline uint64_t load_key(const uint64_t* in, uint64_t i, uint64_t n, 
 }
 }  // namespace
 
-// per-tile digit histogram -> hist[d * ntiles + tile]
+// Where tile `tile`'s count of digit d lives: [digit][tile] rows for the
+// many-tile scan (gqf_sort_scan_rows, one workgroup per row), [tile][digit]
+// up to kScanSmallTiles tiles, where ONE workgroup scans everything and
+// strided rows made every wave access touch 64 lines (15 us per scan).
+__device__ inline uint64_t hist_at(uint32_t d, uint32_t tile, uint32_t ntiles, int radix) {
+  return ntiles <= kScanSmallTiles ? (uint64_t)tile * (uint32_t)radix + d : (uint64_t)d * ntiles + tile;
+}
+
+// per-tile digit histogram -> hist[hist_at(d, tile)]
 template <int ITEMS>
 __device__ __forceinline__ void upsweep_body(const uint64_t* __restrict__ in, uint64_t n, int shift, int bits,
                                              int hash_it, uint64_t mask, int vshift, uint32_t* __restrict__ hist,
@@ -57,7 +65,7 @@ __device__ __forceinline__ void upsweep_body(const uint64_t* __restrict__ in, ui
     }
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < radix; d += kThreads) hist[(uint64_t)d * ntiles + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < radix; d += kThreads) hist[hist_at(d, blockIdx.x, ntiles, radix)] = h[d];
 }
 
 extern "C" __global__ void __launch_bounds__(kThreads)
@@ -145,19 +153,22 @@ gqf_sort_scan_totals(uint32_t* __restrict__ totals, int radix) {
   for (int d = threadIdx.x; d < radix; d += kThreads) totals[d] = s[d];
 }
 
-// Both scans for up to kScanSmallTiles tiles in one workgroup: each digit's
-// row of tile counts becomes its exclusive prefix (as gqf_sort_scan_rows),
+// Both scans for up to kScanSmallTiles tiles in one workgroup, over the
+// [tile][digit] histogram: each digit's column of tile counts becomes its
+// exclusive prefix (loads and stores coalesced across the digits' threads),
 // and the digit totals their exclusive scan (as gqf_sort_scan_totals).
 extern "C" __global__ void __launch_bounds__(kThreads)
 gqf_sort_scan_small(uint32_t* __restrict__ hist, uint32_t ntiles, int radix, uint32_t* __restrict__ totals) {
   __shared__ uint32_t s[kMaxRadix];
   for (int d = threadIdx.x; d < radix; d += kThreads) {
-    uint32_t* row = hist + (uint64_t)d * ntiles;
+    uint32_t v[kScanSmallTiles];
+#pragma unroll
+    for (uint32_t i = 0; i < kScanSmallTiles; i++) v[i] = i < ntiles ? hist[(uint64_t)i * radix + d] : 0u;
     uint32_t run = 0;
-    for (uint32_t i = 0; i < ntiles; i++) {
-      const uint32_t v = row[i];
-      row[i] = run;
-      run += v;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanSmallTiles; i++) {
+      if (i < ntiles) hist[(uint64_t)i * radix + d] = run;
+      run += v[i];
     }
     s[d] = run;
   }
@@ -185,7 +196,7 @@ __device__ __forceinline__ void downsweep_body(const uint64_t* __restrict__ in, 
   for (int d = threadIdx.x; d < radix; d += kThreads) {
 #pragma unroll
     for (int k = 0; k < kWaves; k++) wcnt[k][d] = 0;
-    glob[d] = bin_base[d] + hist[(uint64_t)d * ntiles + blockIdx.x];
+    glob[d] = bin_base[d] + hist[hist_at(d, blockIdx.x, ntiles, radix)];
   }
   __syncthreads();
 

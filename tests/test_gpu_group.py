@@ -206,3 +206,32 @@ def test_group_workgroups_per_cu_keep_results(gpu, oracle):
                 assert filters[f].status() == 0
     finally:
         _close(filters, group)
+
+
+@pytest.mark.parametrize("big", [1 << 24, (1 << 24) + 4099])
+def test_group_segment_sort_limits(gpu, oracle, big):
+    """Segments of at most 2^24 keys are sorted one by one with the
+    two-level sort and tagged afterwards; a larger segment sends the whole
+    group to the tagged LSD sort.  Both give every filter the oracle's
+    image and leave each segment holding its ascending hashes."""
+    from cqf_amd.gqf import GQF, GQFGroup, to_device_u64, to_numpy_u64
+    sizes = [(25, big), (12, 3000), (16, 40000)]
+    _say(f"case group segment sizes {[n for _, n in sizes]}")
+    segs = [oracle.splitmix64(90 + i, n) for i, (_, n) in enumerate(sizes)]
+    offsets = np.cumsum([0] + [n for _, n in sizes]).tolist()
+    filters = [GQF(q) for q, _ in sizes]
+    group = GQFGroup(filters)
+    try:
+        dk = to_device_u64(np.concatenate(segs))
+        group.bulk_insert(dk, offsets)
+        got = to_numpy_u64(dk)
+        for f, (q, n) in enumerate(sizes):
+            seg = segs[f].copy()
+            o = oracle.OracleQF(q)
+            o.bulk_insert(seg)
+            assert np.array_equal(filters[f].export_image(), o.image()), f
+            assert np.array_equal(got[offsets[f]:offsets[f + 1]], seg), f
+            assert filters[f].status() == 0
+            o.close()
+    finally:
+        _close(filters, group)
