@@ -52,7 +52,7 @@ struct Workspace {
   uint32_t* region_list = nullptr;   // non-empty regions of each parity
   uint32_t* list_count = nullptr;    // 2 counts
   uint8_t* wide_flags = nullptr;     // nregions, kept zero by the kernels
-  uint32_t* wide_count = nullptr;    // 1 + GQF_WIDE_LIST: count, then the flagged regions
+  uint32_t* wide_count = nullptr;    // GQF_WIDE_WORDS: count, flagged regions, finished workgroups
   unsigned long long* misses = nullptr;
   // counting reduce (allocated on first use): hash table, counts by
   // compaction index, scan tile sums, control words
@@ -120,8 +120,8 @@ void init_region_workspace(Workspace& w, uint64_t nblocks) {
     dev_alloc(&w.list_count, 2);
     dev_alloc(&w.wide_flags, maxreg + 1);
     GQF_CHECK(hipMemset(w.wide_flags, 0, maxreg + 1));
-    dev_alloc(&w.wide_count, 1 + GQF_WIDE_LIST);
-    GQF_CHECK(hipMemset(w.wide_count, 0, (1 + GQF_WIDE_LIST) * sizeof(uint32_t)));
+    dev_alloc(&w.wide_count, GQF_WIDE_WORDS);
+    GQF_CHECK(hipMemset(w.wide_count, 0, GQF_WIDE_WORDS * sizeof(uint32_t)));
     dev_alloc(&w.misses, 1);
   }
 }

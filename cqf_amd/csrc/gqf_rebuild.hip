@@ -805,13 +805,17 @@ gqf_rebuild_phase_counts(const QF* __restrict__ qfp, const uint64_t* __restrict_
                      sm, icnt);
 }
 
-// Redoes flagged regions of one parity in ascending order with a wider
-// window: 128 blocks, then 256 for a push that outruns that too (a region
-// that needs a wider window has not written anything).  One workgroup;
-// exits at once when nothing was flagged.  Up to GQF_WIDE_LIST flagged
-// regions come from the phase's list, ordered by rank; more than that, from
-// a scan of every region's flag (683 dependent rounds at 2^28 slots, about
-// 40 us whatever the number flagged).
+// Redoes flagged regions of one parity with a wider window: 128 blocks,
+// then 256 for a push that outruns that too (a region that needs a wider
+// window has not written anything).  Regions whose windows could meet are
+// redone in ascending order; regions kWinWide blocks or more apart cannot
+// touch each other's blocks, so the flagged regions are cut into chains of
+// closer ones, and each chain runs on its own workgroup (up to
+// GQF_WIDE_LIST of them).  Up to GQF_WIDE_LIST flagged regions come from
+// the phase's list, ordered by rank; with more, workgroup 0 scans every
+// region's flag in order (683 dependent rounds at 2^28 slots).  Every
+// workgroup exits at once when nothing was flagged; otherwise the last one
+// to finish clears the count for the next phase.
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                  const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
@@ -842,12 +846,28 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
     const uint32_t mine = (uint32_t)lane < nflag ? wide_count[1 + lane] : 0xFFFFFFFFu;
     uint32_t rank = 0;
     for (uint32_t j = 0; j < nflag; j++) rank += (uint32_t)__shfl((int)mine, (int)j, 64) < mine ? 1u : 0u;
-    for (uint32_t k = 0; k < nflag; k++) {
+    auto by_rank = [&](uint32_t k) -> uint64_t {
       const uint64_t bm = __ballot((uint32_t)lane < nflag && rank == k);
-      if (!bm) continue;
-      redo((uint64_t)(uint32_t)__shfl((int)mine, __builtin_ctzll(bm), 64));
+      return bm ? (uint64_t)(uint32_t)__shfl((int)mine, __builtin_ctzll(bm), 64) : ~0ULL;
+    };
+    // regions at least `apart` apart have disjoint windows
+    const uint64_t apart = (kWinWide + qf.region_blocks - 1) / qf.region_blocks;
+    const uint32_t k0 = blockIdx.x;
+    if (k0 < nflag) {
+      uint64_t cur = by_rank(k0);
+      const uint64_t prev = k0 ? by_rank(k0 - 1) : ~0ULL;
+      // workgroup k0 starts a chain when its region is far from the previous one
+      if (cur != ~0ULL && (k0 == 0 || prev == ~0ULL || cur >= prev + apart)) {
+        for (uint32_t k = k0;;) {
+          redo(cur);
+          if (++k >= nflag) break;
+          const uint64_t nxt = by_rank(k);
+          if (nxt == ~0ULL || nxt >= cur + apart) break;
+          cur = nxt;
+        }
+      }
     }
-  } else {
+  } else if (blockIdx.x == 0) {
     for (uint64_t base = (uint64_t)parity; base < qf.nregions; base += 128) {
       const uint64_t r = base + 2ULL * lane;
       const bool f = r < qf.nregions && wide_flags[r];
@@ -859,7 +879,14 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
       }
     }
   }
-  if (lane == 0) *wide_count = 0;
+  // the last workgroup to finish clears the count and the finish counter
+  if (lane == 0) {
+    __threadfence();
+    if (atomicAdd(&wide_count[1 + GQF_WIDE_LIST], 1u) == gridDim.x - 1) {
+      wide_count[1 + GQF_WIDE_LIST] = 0;
+      *(volatile uint32_t*)wide_count = 0;
+    }
+  }
 }
 
 // region_start[r] = first index i with (items[i] & hmask) >= r * (region_blocks * 64 * 256),
@@ -1153,8 +1180,8 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
                          region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
     gqf_prof_end("rebuild_phase", s);
     gqf_prof_begin("rebuild_wide", s);
-    hipLaunchKernelGGL(gqf_rebuild_wide, dim3(1), dim3(64), 0, s, qf_dev, items, counts, region_start, parity, mode,
-                       wide_flags, wide_count);
+    hipLaunchKernelGGL(gqf_rebuild_wide, dim3(GQF_WIDE_LIST), dim3(64), 0, s, qf_dev, items, counts, region_start,
+                       parity, mode, wide_flags, wide_count);
     gqf_prof_end("rebuild_wide", s);
   }
 }
