@@ -80,14 +80,6 @@ gqf_group_tag(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n, int is_h
   }
 }
 
-// the filter index above the hash bits of keys already hashed and sorted
-// per segment
-extern "C" __global__ void __launch_bounds__(256)
-gqf_group_tag_sorted(gqf_group_dev g, uint64_t* __restrict__ keys, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    keys[i] |= (uint64_t)seg_of(g.seg, g.nf, i) << g.tag_shift;
-}
-
 extern "C" __global__ void __launch_bounds__(256)
 gqf_group_untag(uint64_t* __restrict__ keys, uint64_t n, int tag_shift) {
   const uint64_t m = (1ULL << tag_shift) - 1;
@@ -220,10 +212,11 @@ void ensure_sort_space(Group* g, uint64_t n) {
 
 // Keys to be hashed whose every segment fits the two-level sort (at most
 // 2^24 keys, 18..42 hash bits) are sorted segment by segment, hashing on
-// load, and tagged afterwards: equal (filter, hash) values are equal keys,
-// so the unstable two-level sort gives the same array as one LSD sort of
-// the tagged keys (4 passes over 40 bits).  The 256-filter k-mer group's
-// insert call (7.3e8 keys) went from 147 ms to 104 ms.
+// load, its filter's tag ORed in as the bucket sort writes each key: equal
+// (filter, hash) values are equal keys, so the unstable two-level sort
+// gives the same array as one LSD sort of the tagged keys (4 passes over
+// 40 bits).  The 256-filter k-mer group's insert call (7.3e8 keys) went
+// from 147 ms to 104 ms, and to 98 ms without the separate tag pass.
 bool per_segment_sort(const Group* g, int is_hash) {
   if (is_hash) return false;
   for (uint32_t f = 0; f < g->nf; f++) {
@@ -242,9 +235,9 @@ void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t
       if (m == 0) continue;
       const QF& d = g->h_descs[f];
       gqf_launch_sort_two_level(keys + a, g->tmp + a, m, gqf_mask(d.key_bits), (int)d.value_bits,
-                                (int)(d.key_bits + d.value_bits), g->hist, g->totals, g->slow, s);
+                                (int)(d.key_bits + d.value_bits), g->hist, g->totals, g->slow, s, nullptr,
+                                (uint64_t)f << dev.tag_shift);
     }
-    hipLaunchKernelGGL(gqf_group_tag_sorted, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n);
   } else {
     hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
     uint32_t fbits = 0;

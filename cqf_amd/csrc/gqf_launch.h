@@ -33,9 +33,11 @@ struct gqf_region_args {
   uint64_t nregions = 0;
   uint64_t range = 0;
 };
+// tag: ORed into every sorted key on its way out (a filter group's filter
+// index above the hash bits; the region starts are of the untagged hashes)
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
                                uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s,
-                               const gqf_region_args* regions = nullptr);
+                               const gqf_region_args* regions = nullptr, uint64_t tag = 0);
 
 // Counting reduce of duplicate-heavy batches (gqf_agg.hip).  ctrl: 4 words.
 // sample: ctrl[0] = distinct hashes among 4096 keys spread over the batch.
@@ -184,7 +186,7 @@ struct gqf_group_dev {
   const uint64_t* reg_base;   // nf + 1 prefix sums of the region counts (flag index)
   uint32_t nf;
   int tag_shift;              // hash bits below the filter tag
-  uint32_t* counters;         // [1] epoch of the done flags, [2] wide-redo count
+  uint32_t* counters;         // [1] epoch of the done flags, [2] wide-redo count, [3] finished wide workgroups
   uint32_t* done;             // per region: epoch in which it finished
   uint8_t* wide;              // per region: redo with the wide window
 };

@@ -517,7 +517,7 @@ static_assert(kSubMax == 2 * kBThreads, "two sub-buckets per thread in the scan"
 extern "C" __global__ void __launch_bounds__(kBThreads)
 gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n, int low_bits,
                  const uint32_t* __restrict__ starts, uint32_t nbuckets, uint32_t* __restrict__ slow,
-                 gqf_region_args ra) {
+                 gqf_region_args ra, uint64_t tag) {
   __shared__ uint32_t buf[kBucketCap];
   __shared__ uint32_t sub_end[kSubMax];  // counts, then cursors: after the scatter, each sub-bucket's end
   __shared__ uint32_t wsum[kBWaves];
@@ -590,7 +590,7 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
       const uint32_t y = buf[j];
       r += (y < x || (y == x && j < i)) ? 1u : 0u;
     }
-    out[s + st + r] = hi | x;
+    out[s + st + r] = tag | hi | x;
   }
   bucket_region_starts(ra, b, low_bits, s, [&](uint32_t x) {
     const uint32_t d = x >> sb;
@@ -604,7 +604,7 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
 __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out, uint64_t n, int low_bits,
                                                  const uint32_t* __restrict__ starts, uint32_t nbuckets,
                                                  uint32_t* __restrict__ slow, uint32_t b, BucketSmem& sm,
-                                                 const gqf_region_args& ra) {
+                                                 const gqf_region_args& ra, uint64_t tag) {
   const uint64_t s = starts[b];
   const uint64_t e = b + 1 < nbuckets ? (uint64_t)starts[b + 1] : n;
   const uint32_t m = (uint32_t)(e - s);
@@ -627,7 +627,7 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
       __syncthreads();
       src ^= 1;
     }
-    for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = hi | sm.buf[src][i];
+    for (uint32_t i = t; i < m; i += kBThreads) out[s + i] = tag | hi | sm.buf[src][i];
     const uint32_t* sorted = sm.buf[src];
     bucket_region_starts(ra, b, low_bits, s, [&](uint32_t x) {
       uint32_t lo = 0, hi2 = m;
@@ -667,6 +667,10 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
     }
     return lo;
   });
+  if (tag) {
+    __syncthreads();  // the searches above read o untagged
+    for (uint32_t i = t; i < m; i += kBThreads) o[i] |= tag;
+  }
 }
 
 // The flagged buckets (skips the others, and clears the flags it takes):
@@ -677,10 +681,10 @@ __device__ __forceinline__ void sort_flagged_bucket(uint64_t* in, uint64_t* out,
 // one short launch.
 extern "C" __global__ void __launch_bounds__(kBThreads)
 gqf_sort_buckets_slow(uint64_t* in, uint64_t* out, uint64_t n, int low_bits, const uint32_t* __restrict__ starts,
-                      uint32_t nbuckets, uint32_t* __restrict__ slow, gqf_region_args ra) {
+                      uint32_t nbuckets, uint32_t* __restrict__ slow, gqf_region_args ra, uint64_t tag) {
   __shared__ BucketSmem sm;
   for (uint32_t b = blockIdx.x; b < nbuckets; b += gridDim.x) {
-    if (slow[b]) sort_flagged_bucket(in, out, n, low_bits, starts, nbuckets, slow, b, sm, ra);
+    if (slow[b]) sort_flagged_bucket(in, out, n, low_bits, starts, nbuckets, slow, b, sm, ra, tag);
     __syncthreads();
   }
 }
@@ -849,7 +853,7 @@ bool gqf_sort_two_level_fits(uint64_t n, int bits) {
 
 void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64_t mask, int vshift, int bits,
                                uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s,
-                               const gqf_region_args* regions) {
+                               const gqf_region_args* regions, uint64_t tag) {
   const gqf_region_args ra = regions ? *regions : gqf_region_args{};
   // 2^msd buckets of about 12K keys (at most 1024), low bits <= 32
   int msd = 4;
@@ -873,9 +877,9 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
   gqf_prof_end("sort_pass", s);
   gqf_prof_begin("sort_buckets", s);
   hipLaunchKernelGGL(gqf_sort_buckets, dim3(radix), dim3(kBThreads), 0, s, (const uint64_t*)tmp, keys, n, bits - msd,
-                     (const uint32_t*)totals, (uint32_t)radix, slow, ra);
+                     (const uint32_t*)totals, (uint32_t)radix, slow, ra, tag);
   hipLaunchKernelGGL(gqf_sort_buckets_slow, dim3(radix < 256 ? radix : 256), dim3(kBThreads), 0, s, tmp, keys, n, bits - msd,
-                     (const uint32_t*)totals, (uint32_t)radix, slow, ra);
+                     (const uint32_t*)totals, (uint32_t)radix, slow, ra, tag);
   gqf_prof_end("sort_buckets", s);
 }
 

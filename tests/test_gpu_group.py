@@ -211,7 +211,8 @@ def test_group_workgroups_per_cu_keep_results(gpu, oracle):
 @pytest.mark.parametrize("big", [1 << 24, (1 << 24) + 4099])
 def test_group_segment_sort_limits(gpu, oracle, big):
     """Segments of at most 2^24 keys are sorted one by one with the
-    two-level sort and tagged afterwards; a larger segment sends the whole
+    two-level sort, which ORs each filter's tag into its keys; a larger
+    segment sends the whole
     group to the tagged LSD sort.  Both give every filter the oracle's
     image and leave each segment holding its ascending hashes."""
     from cqf_amd.gqf import GQF, GQFGroup, to_device_u64, to_numpy_u64
