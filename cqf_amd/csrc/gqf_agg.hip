@@ -30,17 +30,19 @@
 
 namespace {
 
-constexpr uint64_t kEmpty = ~0ULL - 1;  // no entry: hashes are below 2^48 (and ~0 marks out-of-range keys)
+constexpr uint64_t kEmpty = ~0ULL - 1;  // no entry in the global table: hashes here are below 2^40
 constexpr int kThreads = 256;
 constexpr int kSample = GQF_AGG_SAMPLE;
 constexpr int kSampleCap = 8192;
 constexpr int kTileThreads = 1024;       // threads per workgroup of gqf_agg_tiles
 constexpr int kTileKeys = 131072;        // keys per workgroup of gqf_agg_tiles
-// (2^28 Zipf(1.5) draws, ms: 16K keys / 2048 entries / 256 threads 4.19;
-// 64K / 8192 / 1024 4.13; 128K / 8192 / 1024 3.25; 256K / 8192 / 1024 4.60;
-// 128K / 16384 / 1024 4.35 -- one workgroup per CU)
+// (keys per tile / LDS entries / threads, ms for 2^28 Zipf(1.5) draws:
+// 16K / 2048 / 256 4.19; 64K / 8192 / 1024 4.13; 128K / 8192 / 1024 3.25;
+// 256K / 8192 / 1024 4.60; 128K / 16384 / 1024 4.35, which fits one
+// workgroup per CU)
 constexpr int kWaveRounds = 2;           // hashes a wave merges by ballot before its LDS inserts
-                                         // (1: 4.18 ms, 2: 4.20, 4: 4.49, 6: 4.78 for 2^28 Zipf draws)
+                                         // (1: 3.28 ms, 2: 3.25 with 128K tiles; 1: 4.18, 2: 4.20,
+                                         // 4: 4.49, 6: 4.78 with 16K tiles)
 constexpr int kLoadsPerRound = 8;        // keys each thread loads before inserting them
 constexpr int kLocalLog = 13;
 constexpr int kLocalCap = 1 << kLocalLog;  // LDS table entries
