@@ -284,7 +284,9 @@ gqf_enum_emit(const QF* __restrict__ qfp, uint64_t w_lo, uint64_t nwords,
               uint64_t* __restrict__ hashes, uint64_t* __restrict__ counts, uint64_t* __restrict__ values,
               uint64_t cap) {
   const QF& f = *qfp;
-  const uint64_t rel = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  // the wave's word (block) index, made wave-uniform so that the block's
+  // header words below are scalar loads shared by the 64 lanes
+  const uint64_t rel = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
   const uint64_t i = threadIdx.x & 63;
   if (rel >= nwords || rel > *max_word) return;
   const uint64_t bits = starts[rel];
@@ -293,9 +295,22 @@ gqf_enum_emit(const QF* __restrict__ qfp, uint64_t w_lo, uint64_t nwords,
   if (!((bits >> i) & 1)) return;
   const uint64_t rank = group_base[rel / kThreads] + local_prefix[rel] + (uint64_t)__builtin_popcountll(bits & gqf_mask(i));
   if (rank >= cap) return;
-  const uint64_t p = (w_lo + rel) * 64 + i;
-  const uint64_t q = quotient_of_slot(f, p);
-  const int64_t e = gqf_point::runend_from(f, p, 0);
+  const uint64_t b = w_lo + rel;
+  const uint64_t p = b * 64 + i;
+  const uint64_t off = gqf_point::offset(f, b);
+  const uint64_t runw = gqf_point::run_word(f, b);
+  uint64_t q;
+  if (i >= off) {
+    // past the offset: block b's k-th occupied quotient, k = its run ends in [offset, p)
+    const int k = __builtin_popcountll(runw & gqf_mask(i) & ~gqf_mask(off));
+    const int sel = gqf_select64(gqf_point::occ_word(f, b), k);
+    q = sel < 64 ? b * 64 + (uint64_t)sel : ~0ULL;
+  } else {
+    q = quotient_of_slot(f, p);
+  }
+  // the counter's run ends at the first run end at or after p
+  const uint64_t wend = runw & (~0ULL << i);
+  const int64_t e = wend ? (int64_t)(b * 64) + __builtin_ctzll(wend) : gqf_point::runend_from(f, p, 0);
   if (q == ~0ULL || e < (int64_t)p) {
     atomicOr(f.status, GQF_STATUS_CLUSTER_OVERFLOW);
     return;
