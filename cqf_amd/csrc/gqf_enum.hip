@@ -64,12 +64,28 @@ __device__ inline int64_t walk_items(const QF& f, uint64_t b, uint64_t q_lo, uin
   uint64_t occ = gqf_point::occ_word(f, b);
   int64_t prev = gqf_point::end_before_block(f, b);
   int64_t used = 0;
+  // run-end word of block cb, reloaded only when a run starts or ends in
+  // another block (a load per run made a chain of ~38 dependent loads)
+  uint64_t cb = ~0ULL, cw = 0;
   while (occ) {
     const uint64_t q = b * 64 + (uint64_t)__builtin_ctzll(occ);
     occ &= occ - 1;
     const uint64_t start = (int64_t)q > prev + 1 ? q : (uint64_t)(prev + 1);
-    const int64_t e = gqf_point::runend_from(f, start, 0);
-    if (e < (int64_t)start) return -1;
+    // the run ends at the first run end at or after start
+    uint64_t bj = start >> 6;
+    if (bj >= f.nblocks) return -1;
+    if (bj != cb) {
+      cb = bj;
+      cw = gqf_point::run_word(f, bj);
+    }
+    uint64_t w = cw & (~0ULL << (start & 63));
+    while (!w) {
+      if (++bj >= f.nblocks) return -1;
+      cb = bj;
+      cw = gqf_point::run_word(f, bj);
+      w = cw;
+    }
+    const int64_t e = (int64_t)(bj * 64) + __builtin_ctzll(w);
     prev = e;
     if (q < q_lo || q >= q_hi) continue;
     const uint32_t len = (uint32_t)(e - (int64_t)start + 1);
