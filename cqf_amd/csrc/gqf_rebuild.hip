@@ -134,8 +134,9 @@ struct Items {
 // ---------------------------------------------------------------------------
 // Two-pointer merge of the old counters of one quotient (ascending
 // remainders, old run at st_o .. st_o + old_len - 1 of sin) with its items
-// [it0, it1) (ascending hashes).  Returns the new run length; with WRITE the
-// counters are written from st_n into raw.
+// [it0, it1) (ascending hashes; h_first is item it0's, which the caller
+// has already read).  Returns the new run length; with WRITE the counters
+// are written from st_n into raw.
 //
 // The items of one hash are applied in order, as a sequence of the
 // reference's qf_insert (count + 1, src/gqf.cu:1523-1752) and qf_remove
@@ -147,7 +148,7 @@ struct Items {
 
 template <int W, bool WRITE>
 __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
-                                                   int32_t st_o, uint32_t old_len, int32_t st_n) {
+                                                   int32_t st_o, uint32_t old_len, int32_t st_n, uint32_t h_first) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
   // Common case: one item adding or removing 1 on a run (at most 8 slots) of
@@ -160,7 +161,7 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
   if (it1 == it0 + 1 && old_len <= 8) {
     const int64_t d = src.delta(it0);
     if (d == 1 || d == -1) {
-      const uint32_t r = src.hash(sm.items, it0) & 0xFFu;
+      const uint32_t r = h_first & 0xFFu;
       uint32_t ob[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) ob[k] = (uint32_t)k < old_len ? in[sin_off(st_o + k)] : 0u;
@@ -208,19 +209,21 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
   int64_t n_m = 0, n_s = 0;
   bool have_new = false;
   uint32_t it = it0;
+  uint32_t h_it = h_first;  // hash of item `it`, loaded once (items may be in global memory)
   auto next_new = [&]() {
     have_new = it < it1;
     if (!have_new) return;
-    const uint32_t h0 = src.hash(sm.items, it);
+    const uint32_t h0 = h_it;
     n_rem = h0 & 0xFF;
     n_m = 0;
     n_s = 0;
-    while (it < it1 && src.hash(sm.items, it) == h0) {
+    do {
       const int64_t d = src.delta(it);
       n_s += d;
       n_m = (d > 0 || n_m + d > 0) ? n_m + d : 0;
       it++;
-    }
+      h_it = it < it1 ? src.hash(sm.items, it) : 0u;
+    } while (it < it1 && h_it == h0);
   };
   auto next_old = [&]() {
     oi = o_end + 1;
@@ -308,10 +311,14 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       if (WRITE)
         for (uint32_t k = 0; k < len; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
     } else {
+      // cur is item it's hash (its quotient is qa): each item is loaded once
       const uint32_t it0 = it;
-      while (it < ie && (int32_t)(src.hash(sm.items, it) >> 8) == qa) it++;
-      cur = it < ie ? src.hash(sm.items, it) : 0;
-      len = merge_quotient<W, WRITE>(sm, src, it0, it, st_o, old_len, st_n);
+      const uint32_t h0 = cur;
+      do {
+        it++;
+        cur = it < ie ? src.hash(sm.items, it) : 0u;
+      } while (it < ie && (int32_t)(cur >> 8) == qa);
+      len = merge_quotient<W, WRITE>(sm, src, it0, it, st_o, old_len, st_n, h0);
     }
 
     if (len) {
@@ -578,7 +585,7 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       uint32_t nl = 0;
       if (locate_old_run(sm, nwin, q, st_o, old_len)) {
         const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0);
+        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0, src.hash(sm.items, sm.tit0[t]));
       } else {
         atomicMin(&sm.first_invalid, q >> 6);
         st_o = q;
@@ -713,7 +720,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
     for (uint32_t t = lane; t < nq; t += 64) {
       if (sm.tnl[t] == 0) continue;
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t]);
+      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t],
+                              src.hash(sm.items, sm.tit0[t]));
     }
   }
   __syncthreads();
