@@ -240,16 +240,26 @@ __host__ __device__ static inline uint64_t gqf_table_bytes(uint64_t nblocks) {
 // (r > 0 and top digit >= r).  The digits themselves are recomputed where
 // they are written, least significant first, so no per-lane buffer is
 // needed (a buffer indexed by digit would live in scratch memory).
+// c / base (returned) and c % base (*r) for a counter's digit base: 254
+// for a nonzero remainder, 255 for remainder 0.  Two constant divisors let
+// the compiler multiply by reciprocals instead of running a 64-bit division
+// (the rebuild encodes every counter of count >= 3 this way).
+__host__ __device__ static inline uint64_t gqf_digit_divmod(uint64_t rem, uint64_t c, uint64_t* r) {
+  const uint64_t q = rem ? c / 254 : c / 255;
+  *r = c - q * (rem ? 254 : 255);
+  return q;
+}
+
 __host__ __device__ static inline void gqf_counter_shape(uint64_t rem, uint64_t count, int* n_out, int* pad0_out) {
-  const uint64_t base = rem ? 254 : 255;
   uint64_t c = rem ? count - 3 : count - 4;
   int n = 0;
   uint64_t digit = 0;
   do {
-    digit = c % base + 1;
+    uint64_t r;
+    c = gqf_digit_divmod(rem, c, &r);
+    digit = r + 1;
     if (rem && digit >= rem) digit++;
     n++;
-    c /= base;
   } while (c);
   *n_out = n;
   *pad0_out = (rem && digit >= rem) ? 1 : 0;
@@ -260,13 +270,13 @@ __host__ __device__ static inline void gqf_counter_shape(uint64_t rem, uint64_t 
 template <typename Put>
 __host__ __device__ static inline void gqf_put_counter_digits(uint64_t rem, uint64_t count, uint32_t k0, int n,
                                                               Put put) {
-  const uint64_t base = rem ? 254 : 255;
   uint64_t c = rem ? count - 3 : count - 4;
   for (int i = 0; i < n; i++) {
-    uint64_t digit = c % base + 1;
+    uint64_t r;
+    c = gqf_digit_divmod(rem, c, &r);
+    uint64_t digit = r + 1;
     if (rem && digit >= rem) digit++;
     put(k0 + (uint32_t)(n - 1 - i), digit);
-    c /= base;
   }
 }
 

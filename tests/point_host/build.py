@@ -14,7 +14,8 @@ def build(force=False):
     if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in DEPS):
         return LIB
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp"
+    # a name of its own per process: parallel test workers may build at once
+    tmp = f"{LIB}.{os.getpid()}.tmp"
     subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-fPIC", "-shared", "-std=c++17",
                     "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp], check=True)
     os.replace(tmp, LIB)
