@@ -204,6 +204,21 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
   if (n == 0) return;  // the reference's grid size underflows here
   Filter* f = lookup(qf);
   if (!f) return;
+  // Region starts, sort histograms and reduce counts are 32-bit: a batch of
+  // more than GQF_MAX_PIECE keys is applied as consecutive pieces.  The
+  // table ends up the same (it depends on the multiset only; a bulk_delete
+  // floors each count at 0 in either order), and each piece of `keys` is
+  // left holding its own ascending hashes.
+  if (n > GQF_MAX_PIECE) {
+    uint64_t start[64];
+    const uint32_t np = gqf_batch_pieces(n, GQF_MAX_PIECE, start, 64);
+    if (np + 1 > 64) {  // over 6.7e10 keys: more than any device array here holds
+      fprintf(stderr, "libgqf: batch of %llu keys is too large\n", (unsigned long long)n);
+      return;
+    }
+    for (uint32_t k = 0; k < np; k++) mutate(qf, start[k + 1] - start[k], keys + start[k], flags, mode, reduce, s);
+    return;
+  }
   ensure_workspace(f, n);
   Workspace& w = f->ws;
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
@@ -740,6 +755,10 @@ extern "C" int qf_reserve(QF* qf, uint64_t max_batch) {
   if (!f) return -1;
   ensure_workspace(f, max_batch);
   return f->ws.cap >= max_batch ? 0 : -2;
+}
+
+extern "C" uint32_t qf_batch_pieces(uint64_t n, uint64_t max_piece, uint64_t* starts, uint32_t cap) {
+  return gqf_batch_pieces(n, max_piece, starts, cap);
 }
 
 extern "C" int qf_last_error(void) {

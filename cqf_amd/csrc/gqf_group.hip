@@ -9,9 +9,11 @@
 //            above the hash bits: (f << S) | hash;
 //   sort     one radix sort of all tagged keys (filters come out in
 //            segment order, each sorted by hash);
-//   rebuild  ONE persistent launch over every filter's regions, fed by a
-//            device work queue (gqf_rebuild.hip), plus a wide-redo tail
-//            that exits at once when nothing overflowed;
+//   rebuild  ONE persistent launch over every filter's regions: workgroup
+//            w takes work items w, w + G, w + 2G, ... (even regions of
+//            every filter first, then odd ones that wait for their even
+//            neighbours; gqf_rebuild.hip), plus a wide-redo tail that
+//            exits at once when nothing overflowed;
 //   untag    each segment is left holding its filter's ascending hashes,
 //            as bulk_insert leaves its keys.
 // The sequence is captured once into a hipGraph per shape (key pointer,
@@ -134,6 +136,7 @@ struct Group {
   uint32_t* bslow[kSortBranches] = {};
   uint64_t bcap = 0;
   uint32_t per_cu = 0;  // workgroups per CU of the persistent rebuild (0: default)
+  uint32_t max_polls = GQF_GROUP_MAX_POLLS;  // bound of an odd region's neighbour wait
   // graph cache
   bool use_graph = true;
   hipStream_t cap_stream = nullptr;
@@ -209,6 +212,7 @@ bool refresh(Group* g, const uint64_t* offsets, bool is_hash, gqf_group_dev* dev
   dev->counters = g->d_counters;
   dev->done = g->d_done;
   dev->wide = g->d_wide;
+  dev->max_polls = g->max_polls;
   *n_out = seg[nf];
   return true;
 }
@@ -376,6 +380,14 @@ extern "C" int gqf_group_set_graphs(gqf_group* h, int on) {
   return 0;
 }
 
+extern "C" int gqf_group_set_wait_bound(gqf_group* h, uint32_t max_polls) {
+  Group* g = glookup(h);
+  if (!g) return -1;
+  std::lock_guard<std::mutex> lk(g->mu);
+  g->max_polls = max_polls;
+  return 0;
+}
+
 extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_t* offsets, uint8_t flags,
                                      void* stream) {
   Group* g = glookup(h);
@@ -386,6 +398,10 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   uint64_t n = 0;
   if (!refresh(g, offsets, is_hash, &dev, &n)) return -2;
   if (n == 0) return 0;
+  if (n > GQF_MAX_PIECE) {  // one sort over all segments: 32-bit digit counts
+    fprintf(stderr, "libgqf: group insert of %llu keys (at most 2^30 per call)\n", (unsigned long long)n);
+    return -2;
+  }
   ensure_sort_space(g, n);
   if (!g->tmp || !g->hist) return -3;
   if (per_segment_sort(g, is_hash) && !ensure_branch_space(g)) return -3;
@@ -402,6 +418,7 @@ extern "C" int gqf_group_bulk_insert(gqf_group* h, uint64_t* keys, const uint64_
   key.push_back((uint64_t)(uintptr_t)g->bhist[0]);
   key.push_back((uint64_t)(uintptr_t)g->d_done);
   key.push_back((uint64_t)g->per_cu);
+  key.push_back((uint64_t)g->max_polls);
   for (const QF& d : g->h_descs) key.push_back(d.region_blocks);
   if (!g->exec || key != g->graph_key) {
     if (g->exec) {

@@ -7,8 +7,27 @@
 #include "gqf_device.h"
 
 #define GQF_REGION_BLOCKS 48      // default rebuild region; + 16 push blocks = one wave64
-#define GQF_REGION_BLOCKS_MIN 16  // tunable range (the push window is 64 - region blocks)
+#define GQF_REGION_BLOCKS_MIN 16  // tunable range (the push window is min(64, 2 x region) - region blocks)
 #define GQF_REGION_BLOCKS_MAX 56
+
+// Largest batch one pass of the bulk pipeline takes (its region starts,
+// sort histograms and reduce counts are 32-bit); longer batches are cut
+// into pieces of this size.
+#define GQF_MAX_PIECE (1ULL << 30)
+
+// Cuts n keys into consecutive pieces of at most max_piece keys (max_piece
+// >= 1): piece k is [starts[k], starts[k + 1]).  Returns the number of
+// pieces p (0 for n = 0) and writes starts[0..p] when p + 1 <= cap; p is
+// returned either way, so a caller can size the array.
+static inline uint32_t gqf_batch_pieces(uint64_t n, uint64_t max_piece, uint64_t* starts, uint32_t cap) {
+  if (max_piece == 0) return 0;
+  const uint64_t p = n / max_piece + (n % max_piece ? 1 : 0);
+  if (p + 1 <= cap && starts) {
+    for (uint64_t k = 0; k < p; k++) starts[k] = k * max_piece;
+    starts[p] = n;
+  }
+  return (uint32_t)p;
+}
 
 // status bits reported through qf_get_status()
 #define GQF_STATUS_NO_SPACE QF_STATUS_NO_SPACE                  // a run would end past xnslots; region left unchanged

@@ -189,7 +189,9 @@ struct gqf_group_dev {
   uint32_t* counters;         // [1] epoch of the done flags, [2] wide-redo count, [3] finished wide workgroups
   uint32_t* done;             // per region: epoch in which it finished
   uint8_t* wide;              // per region: redo with the wide window
+  uint32_t max_polls;         // bound of an odd region's wait for an even neighbour's done flag
 };
+#define GQF_GROUP_MAX_POLLS (1u << 22)  // default bound: about 1 s of polls, far above any region's rebuild
 
 // the persistent rebuild over every filter's regions (even items, then odd
 // items that wait for their even neighbours), then the wide redo tail

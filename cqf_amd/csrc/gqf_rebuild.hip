@@ -460,15 +460,24 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
 
 // `tag` is added to every hash of `items` (a filter group sorts all its
 // filters' hashes together, each tagged with its filter above the hash bits).
+//
+// `concurrent`: other waves rebuild the regions of the same parity at the
+// same time (the phase and group kernels).  Regions of one parity are 2 R
+// blocks apart, so the window is then cut to 2 R blocks: region r reads and
+// writes only its own blocks and those of region r + 1, which is of the
+// other parity and not running.  Without the cut, a region of R < 32 blocks
+// would push into region r + 2 while that region's wave rebuilds it.  A push
+// the cut window cannot absorb is redone by the serial wide pass.
 template <int W>
 __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
-                              Smem<W>& sm, uint32_t* lcnt, uint64_t tag = 0) {
+                              Smem<W>& sm, uint32_t* lcnt, bool concurrent, uint64_t tag = 0) {
   constexpr int kRounds = W / 64;
   const int lane = lane_id();
   const int R = (int)qf.region_blocks;
   const uint64_t B0 = r * (uint64_t)R;
-  const int nwin = (int)min((uint64_t)W, qf.nblocks - B0);
+  const uint64_t wmax = concurrent && 2 * R < W ? (uint64_t)(2 * R) : (uint64_t)W;
+  const int nwin = (int)min(wmax, qf.nblocks - B0);
   const int nreg = (int)min((uint64_t)R, qf.nblocks - B0);
   const int64_t win_base = (int64_t)(B0 * 64);
   const bool table_end_in_window = B0 + (uint64_t)nwin >= qf.nblocks;
@@ -780,7 +789,7 @@ __device__ __forceinline__ void rebuild_phase_item(const QF* __restrict__ qfp, c
   if (r >= qf.nregions) return;
   const uint64_t s = region_start[r], e = region_start[r + 1];
   if (s == e) return;
-  const int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm, lcnt);
+  const int res = rebuild_region<kWinFast>(qf, r, items, counts, s, e, mode, sm, lcnt, true);
   if (lane_id() == 0) {
     if (res == kNeedWide) {
       wide_flags[r] = 1;
@@ -838,9 +847,9 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   const int lane = lane_id();
   auto redo = [&](uint64_t rr) {
     const uint64_t s = region_start[rr], e = region_start[rr + 1];
-    int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt);
+    int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt, false);
     __syncthreads();
-    if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt);
+    if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt, false);
     __syncthreads();
     if (lane == 0) {
       wide_flags[rr] = 0;
@@ -1010,8 +1019,9 @@ extern "C" __global__ void gqf_region_lists_reset(uint32_t* __restrict__ list_co
 // (whose windows overlap its own) are done.  Every workgroup finishes its
 // even items before it reaches any odd item, and even items wait for
 // nothing, so each awaited item belongs to a resident workgroup that gets
-// to it: the wait always ends (it is bounded anyway and reported in the
-// status if the bound runs out).  Done flags carry the epoch of the call,
+// to it: the wait always ends.  It is bounded anyway (max_polls, which the
+// host sets): a region whose wait runs out is flagged for the wide tail
+// instead of being rebuilt.  Done flags carry the epoch of the call,
 // so they never need clearing.  Writes are published with an agent-scope
 // release before the flag and read after an acquire, since the two regions
 // may run on different XCDs.
@@ -1039,8 +1049,8 @@ __device__ inline uint32_t item_filter(const uint64_t* __restrict__ base, uint32
 // Bounded wait for a region's completion flag, executed by the whole wave
 // with a wave-uniform exit (readfirstlane of the polled value), so the
 // loop cannot be split between lanes; false if the bound ran out.
-__device__ inline bool wait_done(const uint32_t* flag, uint32_t epoch) {
-  for (uint32_t spin = 0; spin < (1u << 22); spin++) {
+__device__ inline bool wait_done(const uint32_t* flag, uint32_t epoch, uint32_t max_polls) {
+  for (uint32_t spin = 0; spin < max_polls; spin++) {
     const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (__builtin_amdgcn_readfirstlane(v) == epoch) return true;
     __builtin_amdgcn_s_sleep(4);
@@ -1083,16 +1093,20 @@ gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode)
     const uint64_t r = 2 * (k - base[f]) + (odd ? 1 : 0);
     const QF qf = g.descs[f];
     const uint64_t flag = g.reg_base[f] + r;
+    // an odd region whose wait ran out is not rebuilt here (its neighbour
+    // may still be writing its window): it goes to the wide tail, which
+    // runs after this kernel has finished
+    bool ok = true;
     if (odd) {
-      bool ok = wait_done(&g.done[flag - 1], epoch);
-      if (r + 1 < qf.nregions) ok = wait_done(&g.done[flag + 1], epoch) && ok;
+      ok = wait_done(&g.done[flag - 1], epoch, g.max_polls);
+      if (ok && r + 1 < qf.nregions) ok = wait_done(&g.done[flag + 1], epoch, g.max_polls);
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (!ok && lane == 0) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
     }
     uint64_t s, e, tag;
     region_hash_bounds(g, qf, f, r, items, s, e, tag);
     int res = kOk;
-    if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, nullptr, tag);
+    if (!ok) res = s < e ? kNeedWide : kOk;
+    else if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, nullptr, true, tag);
     __syncthreads();
     if (lane == 0) {
       if (res == kNeedWide) {
@@ -1131,9 +1145,9 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
         m &= m - 1;
         uint64_t s, e, tag;
         region_hash_bounds(g, qf, f, rr, items, s, e, tag);
-        int res = rebuild_region<kWinMid>(qf, rr, items, nullptr, s, e, mode, smm, nullptr, tag);
+        int res = rebuild_region<kWinMid>(qf, rr, items, nullptr, s, e, mode, smm, nullptr, false, tag);
         __syncthreads();
-        if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, nullptr, tag);
+        if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, nullptr, s, e, mode, sm, nullptr, false, tag);
         __syncthreads();
         if (lane == 0) {
           g.wide[f0 + rr] = 0;

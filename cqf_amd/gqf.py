@@ -39,7 +39,8 @@ EXPORTS = [
     "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
-    "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu",
+    "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu", "gqf_group_set_wait_bound",
+    "qf_batch_pieces",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
     "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
     "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust",
@@ -129,6 +130,10 @@ def load_library(path=LIB_PATH):
     L.gqf_group_set_workgroups_per_cu.restype = ctypes.c_int
     L.gqf_group_get_workgroups_per_cu.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
     L.gqf_group_get_workgroups_per_cu.restype = ctypes.c_int
+    L.gqf_group_set_wait_bound.argtypes = [_vp, ctypes.c_uint32]
+    L.gqf_group_set_wait_bound.restype = ctypes.c_int
+    L.qf_batch_pieces.argtypes = [_u64, _u64, ctypes.POINTER(_u64), ctypes.c_uint32]
+    L.qf_batch_pieces.restype = ctypes.c_uint32
     L.qf_malloc_device_kv.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
     L.qf_enumerate.argtypes = [_vp, _u64, _u64, ctypes.c_int, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp]
     L.qf_enumerate.restype = ctypes.c_int
@@ -481,10 +486,27 @@ class GQFGroup:
         if rc != 0:
             raise ValueError(f"gqf_group_set_workgroups_per_cu({per_cu}) rc={rc}")
 
+    def set_wait_bound(self, max_polls):
+        """Bound on an odd region's neighbour wait in the persistent rebuild
+        (regions whose wait runs out are redone by the wide pass)."""
+        rc = self._lib.gqf_group_set_wait_bound(self.handle, int(max_polls))
+        if rc != 0:
+            raise RuntimeError(f"gqf_group_set_wait_bound rc={rc}")
+
     def close(self):
         if self.handle is not None and self.handle.value:
             self._lib.gqf_group_destroy(self.handle)
         self.handle = None
+
+
+def batch_pieces(n, max_piece):
+    """Start indices (p + 1 of them) of the pieces a bulk mutation of n keys
+    is cut into, pieces of at most max_piece keys."""
+    L = load_library()
+    p = int(L.qf_batch_pieces(n, max_piece, None, 0))
+    arr = (_u64 * (p + 1))()
+    L.qf_batch_pieces(n, max_piece, arr, p + 1)
+    return [int(x) for x in arr]
 
 
 def autotune(nbits, load):

@@ -10,7 +10,8 @@
  *   - bulk calls are asynchronous on the null stream (src/gqf.cu:3184-3419);
  *     the *_wrapper calls synchronise and return a host value.
  * Differences, all additive: nvals == 0 is a no-op (the reference underflows
- * its grid size); table-full and cluster-overflow conditions are recorded
+ * its grid size); a mutation of more than 2^30 keys runs as pieces of 2^30
+ * (see qf_batch_pieces); table-full and cluster-overflow conditions are recorded
  * in a status word (qf_get_status) instead of being printed and ignored;
  * stream-taking variants and image export/import exist for tests and the
  * serving loop.  No signature carries a HIP or torch type: streams are
@@ -121,8 +122,8 @@ int qf_reset_device(QF *qf, void *stream);
  * later bulk calls allocate nothing */
 int qf_reserve(QF *qf, uint64_t max_batch);
 /* tuning (0 keeps a value): blocks per rebuild region, 16..56 (the push
- * window is 64 minus that), and radix digit bits of the bulk sort, 6..10;
- * the results do not depend on them, only the speed does */
+ * window is min(64, 2 x region) minus that), and radix digit bits of the
+ * bulk sort, 6..10; the results do not depend on them, only the speed does */
 int qf_set_tuning(QF *qf, uint32_t region_blocks, uint32_t sort_digit_bits);
 /* stream-ordered variants of the bulk calls */
 void bulk_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
@@ -175,7 +176,8 @@ int qf_bulk_get_seq_adjust(QF *qf, uint64_t n, const uint64_t *keys, uint32_t nr
  * into a hipGraph per shape and replayed; gqf_group_bulk_get fills
  * returns[i] with the count of vals[i] in its segment's filter, in one
  * launch.  Stream-ordered; a call whose segments or filter tuning differ
- * from the previous one synchronises first. */
+ * from the previous one synchronises first.  A group insert takes at most
+ * 2^30 keys in all (-2 above that). */
 typedef struct gqf_group gqf_group;
 int gqf_group_create(gqf_group **g, QF *const *filters, uint32_t nfilters);
 void gqf_group_destroy(gqf_group *g);
@@ -193,6 +195,11 @@ int gqf_autotune(int nbits, double load, uint32_t *region_blocks, uint32_t *sort
  * scratch filters of the same sizes (candidates 4..16 up to what the GPU
  * keeps resident); synchronises */
 int gqf_group_autotune(gqf_group *g, const uint64_t *offsets);
+/* bound on the polls of an odd region's wait for its even neighbours in
+ * the persistent rebuild (default 2^22).  A region whose wait runs out is
+ * not rebuilt in that launch but redone by the serial wide pass after it,
+ * so results never depend on the bound; 0 sends every odd region there. */
+int gqf_group_set_wait_bound(gqf_group *g, uint32_t max_polls);
 /* workgroups per CU of the persistent rebuild (0: the default, 8); above
  * what the GPU keeps resident -> -2.  The getter reports the value used. */
 int gqf_group_set_workgroups_per_cu(gqf_group *g, uint32_t per_cu);
@@ -266,6 +273,14 @@ int64_t qf_resize_device(QF *qf, int new_nbits);
 int64_t qf_merge_device(QF *qfa, QF *qfb, QF *qfc);
 /* qfr += every item of the nqf filters (qf_multi_merge, src/gqf.cu:3943-4008) */
 int64_t qf_multi_merge_device(QF *const *qfs, int nqf, QF *qfr);
+
+/* How a bulk mutation of n keys is cut: pieces of at most max_piece keys
+ * (the library uses 2^30), piece k = [starts[k], starts[k + 1]).  Returns
+ * the number of pieces p and fills starts[0..p] if cap >= p + 1.  A batch of
+ * more than 2^30 keys is applied piece by piece; each piece of `keys` is
+ * then left holding its own ascending hashes (the reference sorts the whole
+ * array, src/gqf.cu:3209).  Pure host arithmetic. */
+uint32_t qf_batch_pieces(uint64_t n, uint64_t max_piece, uint64_t *starts, uint32_t cap);
 
 /* last HIP error seen by the library (0 = none); clears it */
 int qf_last_error(void);

@@ -32,3 +32,23 @@ def test_library_exports_every_declared_symbol():
 def test_python_mirror_lists_all_exports():
     from cqf_amd import gqf
     assert set(_declared_functions()) == set(gqf.EXPORTS)
+
+
+def test_batch_pieces_arithmetic():
+    """How a bulk mutation longer than the pipeline's 32-bit indices allow is
+    cut: consecutive pieces of at most max_piece keys that cover [0, n)
+    exactly (host arithmetic only)."""
+    from cqf_amd import gqf
+    P = 1 << 30
+    assert gqf.batch_pieces(0, P) == [0]
+    assert gqf.batch_pieces(1, P) == [0, 1]
+    assert gqf.batch_pieces(P, P) == [0, P]
+    assert gqf.batch_pieces(P + 1, P) == [0, P, P + 1]
+    n = (1 << 32) + 12345  # 2^32 keys or more would wrap a 32-bit region start
+    st = gqf.batch_pieces(n, P)
+    assert st[0] == 0 and st[-1] == n and len(st) == 6
+    assert all(0 < b - a <= P for a, b in zip(st, st[1:]))
+    assert all(b - a < (1 << 31) for a, b in zip(st, st[1:]))
+    for n, m in ((10, 3), (9, 3), (7, 1), (5, 100)):
+        st = gqf.batch_pieces(n, m)
+        assert st == list(range(0, n, m)) + [n]
