@@ -4,6 +4,7 @@
 // header gqf_device.h.
 #pragma once
 
+#define GQF_NO_POINT_BULK_KERNEL  // the library launches its own point kernels (gqf_point.hip)
 #include "gqf_device.h"
 
 #define GQF_REGION_BLOCKS 48      // default rebuild region; + 16 push blocks = one wave64

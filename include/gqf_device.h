@@ -727,4 +727,21 @@ __device__ static inline uint64_t point_query_concurrent(QF* qf, uint64_t key, u
   return c;
 }
 
+// The reference's point_bulk_insert kernel (include/gqf.cuh:136,
+// src/gqf.cu:3153-3162): thread i inserts one instance of hashes[i] with
+// point_insert and flags 0, so the items are hashed like any key.  The
+// reference's callers launch it as point_bulk_insert<<<(n - 1) / 32 + 1, 32>>>
+// (include/point_wrapper.cuh:63); any launch shape works.  A full table
+// sets QF_STATUS_POINT_FULL (the reference's assert compares a qf_returns
+// with QF_NO_SPACE, which it never equals).  Defined here, in the header, so
+// the caller's launch compiles unchanged: a kernel cannot be exported
+// across a shared library.  GQF_NO_POINT_BULK_KERNEL leaves it out.
+#ifndef GQF_NO_POINT_BULK_KERNEL
+static __global__ void point_bulk_insert(QF* qf, uint64_t* hashes, uint64_t nitems) {
+  const uint64_t tid = threadIdx.x + (uint64_t)blockIdx.x * blockDim.x;
+  if (tid >= nitems) return;
+  (void)point_insert(qf, hashes[tid], 0, 0);
+}
+#endif
+
 #endif /* GQF_DEVICE_H */

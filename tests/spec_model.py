@@ -61,20 +61,37 @@ def sizes(nbits):
 
 def build_image(nbits, multiset):
     """multiset: dict hash -> count (hash < 2^(nbits+8)).  Returns bytes."""
-    nslots, xnslots, nblocks = sizes(nbits)
-    slots = [0] * (nblocks * 64)
-    runend = [0] * (nblocks * 64)
-    occ = [0] * (nblocks * 64)
+    return image_from_runs(nbits, runs_of(multiset))
+
+
+def runs_of(multiset):
+    """{quotient: slot bytes of its run}: the counters of its remainders in
+    ascending remainder order."""
     by_q = {}
     for h, c in multiset.items():
         if c > 0:
             by_q.setdefault(h >> 8, []).append((h & 0xFF, c))
+    out = {}
+    for q, rcs in by_q.items():
+        run = []
+        for rem, c in sorted(rcs):
+            run += encode(rem, c)
+        out[q] = run
+    return out
+
+
+def image_from_runs(nbits, runs):
+    """The image whose quotient q holds the slot bytes runs[q] (any bytes:
+    the layout does not read them), each run at max(q, end of the previous
+    run + 1).  Returns bytes."""
+    nslots, xnslots, nblocks = sizes(nbits)
+    slots = [0] * (nblocks * 64)
+    runend = [0] * (nblocks * 64)
+    occ = [0] * (nblocks * 64)
     prev_end = -1
     ends = []  # (quotient, end) in quotient order
-    for q in sorted(by_q):
-        run = []
-        for rem, c in sorted(by_q[q]):
-            run += encode(rem, c)
+    for q in sorted(q for q, r in runs.items() if r):
+        run = list(runs[q])
         start = max(q, prev_end + 1)
         end = start + len(run) - 1
         if end >= xnslots:
@@ -101,6 +118,27 @@ def build_image(nbits, multiset):
         img[base + 9:base + 17] = r.to_bytes(8, "little")
         img[base + 17:base + 81] = bytes(slots[64 * b:64 * b + 64])
     return bytes(img)
+
+
+def insert_not_exists(runs, h):
+    """A not-exists insert of hash h on {quotient: run bytes}, searched slot
+    by slot as the format's readers would NOT (they walk counters): skip the
+    run's slots while the slot is below the remainder; past the run end,
+    append; at a larger slot, insert the remainder before it; at an equal
+    slot, report it found and change nothing.  A quotient without a run
+    gets the run [remainder].  Returns "inserted" or "found"."""
+    q, r = h >> 8, h & 0xFF
+    run = runs.get(q)
+    if not run:
+        runs[q] = [r]
+        return "inserted"
+    i = 0
+    while i < len(run) and run[i] < r:
+        i += 1
+    if i < len(run) and run[i] == r:
+        return "found"
+    run.insert(i, r)
+    return "inserted"
 
 
 def used_slots(nbits, multiset):

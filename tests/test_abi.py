@@ -52,3 +52,16 @@ def test_batch_pieces_arithmetic():
     for n, m in ((10, 3), (9, 3), (7, 1), (5, 100)):
         st = gqf.batch_pieces(n, m)
         assert st == list(range(0, n, m)) + [n]
+
+
+def test_drop_in_caller_library_loads():
+    """The reference-style caller (tests/drop_in) built by build() loads
+    with libgqf.so and exports its entry points (no compute)."""
+    from cqf_amd import gqf
+    from tests.drop_in import build as B
+    gqf.load_library()
+    if not os.path.exists(B.LIB):
+        B.build()
+    lib = ctypes.CDLL(B.LIB)
+    for n in ("dropin_gqf_init", "dropin_point_bulk_insert", "dropin_count_kmers", "dropin_gqf_bulk_get"):
+        assert hasattr(lib, n)

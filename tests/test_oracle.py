@@ -303,3 +303,171 @@ def test_insert_not_exists_semantics(oracle):
     assert (ret, val) == (1, 0)                # found; returnedVal is always 0 (H13)
     assert o.count(k) == 1
     o.close()
+
+
+# ---------------------------------------------------------------------------
+# insert1_if_not_exists (src/gqf.cu:1364-1518), pinned two ways:
+#   1. on count-1 tables the slot-wise search is the counter-wise one, so
+#      not-exists inserts of any hash sequence leave exactly the canonical
+#      image of the SET of hashes (tests/spec_model.build_image);
+#   2. on runs with multi-slot counters the search is slot-wise: hand-derived
+#      run bytes from the reference lines, and a model of the search on run
+#      byte lists (tests/spec_model.insert_not_exists) whose layout is the
+#      spec model's, over random mixed tables.
+
+KEY_IS_HASH = 0x08
+INSERTED, FOUND = 0, 1
+
+
+def test_not_exists_count1_tables_are_the_canonical_set(oracle):
+    import random
+    rnd = random.Random(11)
+    for nbits, n in ((8, 230), (10, 900), (12, 3800)):
+        o = oracle.OracleQF(nbits)
+        rng = 1 << (nbits + 8)
+        hs = [rnd.randrange(rng) for _ in range(n)]
+        hs += [hs[rnd.randrange(len(hs))] for _ in range(n // 4)]  # repeats: FOUND
+        seen = set()
+        for h in hs:
+            ret, val = o.insert_not_exists(h, KEY_IS_HASH)
+            if h in seen:
+                assert (ret, val) == (FOUND, 0), h  # returnedVal is always 0 (H13)
+            else:
+                assert (ret, val) == (INSERTED, 77), h  # returnedVal untouched
+            seen.add(h)
+        want = S.build_image(nbits, {h: 1 for h in seen})
+        assert o.image().tobytes() == want, nbits
+        # a found key changes nothing
+        before = o.image()
+        for h in list(seen)[:50]:
+            assert o.insert_not_exists(h, KEY_IS_HASH)[0] == FOUND
+        assert np.array_equal(o.image(), before)
+        o.close()
+
+
+def _mixed_table(oracle, nbits, runs_spec):
+    """A table holding {quotient: [(remainder, count), ...]} (inserted with
+    counts through the oracle, whose images the spec model pins), and its
+    run bytes."""
+    o = oracle.OracleQF(nbits)
+    ms = {}
+    for q, rcs in runs_spec.items():
+        for r, c in rcs:
+            h = (q << 8) | r
+            assert o.insert(h, c, KEY_IS_HASH) >= 0
+            ms[h] = ms.get(h, 0) + c
+    runs = S.runs_of(ms)
+    assert o.image().tobytes() == S.image_from_runs(nbits, runs)
+    return o, runs
+
+
+def test_not_exists_multislot_known_answers(oracle):
+    """Hand-derived cases on a count-300 counter of remainder 5 at quotient
+    126 (the run [5, 2, 45, 5]: 297 = 1 x 254 + 43, digits 1 + 1 = 2 and
+    43 + 1 = 44, which is >= 5 and becomes 45) and a count-3 counter of
+    remainder 0 at quotient 127 ([0, 0, 0]), inside a cluster that crosses
+    into block 2, so each insert shifts the runs after it and moves block
+    2's offset."""
+    assert S.encode(5, 300) == [5, 2, 45, 5]
+    assert S.encode(0, 3) == [0, 0, 0]
+    spec = {q: [(17, 1), (90, 1)] for q in range(118, 126)}
+    spec[126] = [(5, 300)]
+    spec[127] = [(0, 3)]
+    spec.update({q: [(33, 1), (60, 1), (61, 1)] for q in range(128, 134)})
+    cases = [
+        # :1413-1419 skips 5 and 2 (both < 7) and stops at 45 >= 7; 45 != 7,
+        # so :1431-1436 (operation 2) puts 7 before 45, inside the counter
+        ((126 << 8) | 7, INSERTED, {126: [5, 2, 7, 45, 5]}),
+        # the skip stops at 45 == 45: :1437-1444 returns QF_ITEM_FOUND (a
+        # digit, not a remainder), nothing moves
+        ((126 << 8) | 45, FOUND, {}),
+        # every slot is below 200: the index passes the run end, :1425-1429
+        # (operation 1) appends after it and moves the run end bit
+        ((126 << 8) | 200, INSERTED, {126: [5, 2, 45, 5, 200]}),
+        # the first slot, 5, is >= 3 and != 3: operation 2 at the run start
+        ((126 << 8) | 3, INSERTED, {126: [3, 5, 2, 45, 5]}),
+        # the zero counter: slot 0 == 0 at the run start, found
+        ((127 << 8) | 0, FOUND, {}),
+        # 1 is above every slot of [0, 0, 0]: appended
+        ((127 << 8) | 1, INSERTED, {127: [0, 0, 0, 1]}),
+        # an empty quotient past the cluster (slot 140 is used by it): a new
+        # run at the cluster's end (operation 0, :1447-1466)
+        ((134 << 8) | 9, INSERTED, {134: [9]}),
+    ]
+    for h, ret, change in cases:
+        o, runs = _mixed_table(oracle, 10, spec)
+        before = o.image()
+        got, val = o.insert_not_exists(h, KEY_IS_HASH)
+        assert got == ret, hex(h)
+        if ret == FOUND:
+            assert val == 0
+            assert np.array_equal(o.image(), before), hex(h)
+        else:
+            runs.update(change)
+            assert o.image().tobytes() == S.image_from_runs(10, runs), hex(h)
+        o.close()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_not_exists_slotwise_model_on_random_mixed_tables(oracle, seed):
+    """Random tables with counters of counts 1..70000 (multi-slot), then 300
+    not-exists inserts, most aimed at occupied quotients: return codes and
+    the images equal the slot-wise model on run bytes."""
+    import random
+    rnd = random.Random(seed)
+    nbits = 10
+    qs = rnd.sample(range(1 << nbits), 120)
+    spec = {}
+    for q in qs:
+        rems = sorted(rnd.sample(range(256), rnd.choice((1, 1, 2, 3))))
+        spec[q] = [(r, rnd.choice((1, 1, 2, 3, 4, 17, 254, 300, 70000))) for r in rems]
+    o, runs = _mixed_table(oracle, nbits, spec)
+    for i in range(300):
+        q = rnd.choice(qs) if i % 4 else rnd.randrange(1 << nbits)
+        h = (q << 8) | rnd.randrange(256)
+        want = S.insert_not_exists(runs, h)
+        got, val = o.insert_not_exists(h, KEY_IS_HASH)
+        assert got == (FOUND if want == "found" else INSERTED), (i, hex(h))
+        if i % 20 == 0 or want == "inserted":
+            assert o.image().tobytes() == S.image_from_runs(nbits, runs), (i, hex(h))
+    o.close()
+
+
+def test_remove_path_truncates_offsets_h5(oracle):
+    """Hazard H5.  The reference's remove path stores each re-derived block
+    offset in the u8 field without saturating (src/gqf.cu:1172-1190), while
+    its inserts saturate at 255 (:1748-1753).  Quotients 0..3 with 200
+    remainders each fill slots 0..799: blocks 1..8 have exact offsets
+    736 - 64 (b - 1) > 255 (stored 255), block 9 has 224.  Removing one key
+    ends the runs of quotients < 64 at 798, and the loop writes, derived by
+    hand from those lines:
+      block 1: 798 - 127 + 64 = 735, stored as 735 mod 256 = 223;
+      block 2: run_end(127) reads block 1's 223 > 63 with no occupied
+               quotient, so it is 64 + 223 - 1 = 286: offset 286 - 127 = 159;
+      block 3: 286 - 191 = 95;  block 4: 286 - 255 = 31;
+      block 5: run_end(319) reads 31 <= 63: 319, which is in block 4, so 0;
+      blocks 6..12: 0 likewise (their stored 255 / 224 / 160 / 96 / 32 differ
+               from 0); block 13 already holds 0 and the loop stops.
+    Slots, occupieds and run ends equal the canonical image; only those
+    offset bytes differ from it (255 x 8, 223, 159, 95, 31).  The device
+    table keeps exact offsets and exports the canonical image instead."""
+    o = oracle.OracleQF(10)
+    ms = {}
+    for q in range(4):
+        for r in range(200):
+            h = (q << 8) | r
+            assert o.insert(h, 1, KEY_IS_HASH) >= 0
+            ms[h] = 1
+    img = o.image()
+    assert img.tobytes() == S.build_image(10, ms)
+    assert [int(img[81 * b]) for b in range(14)] == [0] + [255] * 8 + [224, 160, 96, 32, 0]
+    h = (1 << 8) | 7
+    assert o.remove(h, 1, KEY_IS_HASH) >= 0
+    del ms[h]
+    img = o.image()
+    canon = np.frombuffer(S.build_image(10, ms), dtype=np.uint8)
+    assert [int(img[81 * b]) for b in range(14)] == [0, 223, 159, 95, 31] + [0] * 9
+    assert [int(canon[81 * b]) for b in range(14)] == [0] + [255] * 8 + [223, 159, 95, 31, 0]
+    diff = np.nonzero(img != canon)[0]
+    assert diff.size and all(p % 81 == 0 for p in diff)  # offset bytes only
+    o.close()
