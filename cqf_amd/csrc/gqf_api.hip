@@ -306,7 +306,7 @@ inline uint32_t select64_host(uint64_t w, uint32_t r) {  // position of the r-th
 // Rejects an image whose offset bytes disagree with that (min(offset, 255))
 // or whose run ends and occupied quotients do not pair up.
 // `out` receives the whole table allocation (gqf_table_bytes), slack zeroed.
-bool image_to_device_layout(const uint8_t* img, uint64_t nblocks, std::vector<uint8_t>& out) {
+bool image_to_device_layout(const uint8_t* img, uint64_t nblocks, uint64_t xnslots, std::vector<uint8_t>& out) {
   out.assign(gqf_table_bytes(nblocks), 0);
   uint64_t occ_before = 0;   // occupied quotients < 64 b
   uint64_t cb = 0;           // run-end cursor block
@@ -333,6 +333,9 @@ bool image_to_device_layout(const uint8_t* img, uint64_t nblocks, std::vector<ui
     memcpy(dst, src + 1, 16);  // occupieds, runends (little endian both sides)
     const uint32_t off32 = (uint32_t)off;
     memcpy(dst + 16, &off32, 4);
+    const uint64_t lim = xnslots - b * 64;
+    const uint64_t used = gqf_used_mask(occ, load_le64(src + 9), off32, lim < 64 ? (uint32_t)lim : 64u);
+    memcpy(dst + 24, &used, 8);
     memcpy(dst + GQF_SLOT_BASE, src + 17, 64);
     occ_before += (uint64_t)__builtin_popcountll(occ);
     total_ends += (uint64_t)__builtin_popcountll(load_le64(src + 9));
@@ -681,7 +684,7 @@ extern "C" int qf_import_image(QF* qf, const void* host_src, uint64_t nbytes) {
   const uint64_t sz = f->host.nblocks * GQF_REF_BLOCK_BYTES;
   if (nbytes != sz) return -2;
   std::vector<uint8_t> dev_img;
-  if (!image_to_device_layout((const uint8_t*)host_src, f->host.nblocks, dev_img)) {
+  if (!image_to_device_layout((const uint8_t*)host_src, f->host.nblocks, f->host.xnslots, dev_img)) {
     fprintf(stderr, "libgqf: qf_import_image: inconsistent block image (offsets / run ends)\n");
     return -4;
   }
@@ -1066,6 +1069,23 @@ extern "C" int qf_get_stats(QF* qf, uint64_t* nelts, uint64_t* ndistinct_elts, u
   if (nelts) *nelts = st[1];
   if (noccupied_slots) *noccupied_slots = st[2];
   return 0;
+}
+
+extern "C" int qf_check_layout(QF* qf, uint64_t* bad_blocks) {
+  Filter* f = lookup(qf);
+  if (!f || !bad_blocks) return -1;
+  unsigned long long* bad = nullptr;
+  dev_alloc(&bad, 1);
+  if (!bad) return -3;
+  GQF_CHECK(hipDeviceSynchronize());
+  GQF_CHECK(hipMemset(bad, 0, sizeof(unsigned long long)));
+  gqf_launch_check_used(f->dev, f->host.nblocks, bad, nullptr);
+  unsigned long long h = 0;
+  const hipError_t e = hipMemcpy(&h, bad, sizeof(h), hipMemcpyDeviceToHost);
+  note_error(e, "qf_check_layout");
+  dev_free(bad);
+  *bad_blocks = (uint64_t)h;
+  return e == hipSuccess ? 0 : -3;
 }
 
 extern "C" int qf_cluster_census(QF* qf, uint64_t* lengths, uint64_t cap, uint64_t* hist, uint32_t nbins,

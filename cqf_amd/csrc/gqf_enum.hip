@@ -349,29 +349,25 @@ gqf_enum_emit(const QF* __restrict__ qfp, uint64_t w_lo, uint64_t nwords,
 // (offset_lower_bound != 0, src/gqf.cu:681-705).
 static __device__ inline uint64_t used_mask(const QF& f, uint64_t b) {
   const gqf_header h = gqf_load_header(f, b);
-  uint64_t used;
-  if (h.offset >= 64) {
-    used = ~0ULL;
-  } else {
-    const int off = (int)h.offset;
-    used = gqf_mask((uint64_t)off);
-    int p = __builtin_popcountll(h.occupieds & gqf_mask((uint64_t)off));
-    int r = 0;
-    for (int i = off; i < 64; i++) {
-      p += (int)((h.occupieds >> i) & 1);
-      if (p > r) used |= 1ULL << i;
-      r += (int)((h.runends >> i) & 1);
-    }
-  }
   const uint64_t lim = f.xnslots - b * 64;
-  if (lim < 64) used &= gqf_mask(lim);
-  return used;
+  return gqf_used_mask(h.occupieds, h.runends, h.offset, lim < 64 ? (uint32_t)lim : 64u);
 }
 
 // whether slot 64 b - 1 is used: earlier runs reach into block b, or a run
 // ends exactly there
 static __device__ inline uint64_t prev_slot_used(const QF& f, uint64_t b) {
   return b > 0 && (gqf_point::offset(f, b) > 0 || (gqf_load_runends(f, b - 1) >> 63)) ? 1 : 0;
+}
+
+// Blocks whose stored used-slot mask (bytes 24..31) differs from the one
+// their header implies: one thread per block, one atomic per mismatch.
+extern "C" __global__ void __launch_bounds__(kThreads)
+gqf_check_used(const QF* __restrict__ qfp, unsigned long long* __restrict__ bad) {
+  const QF& f = *qfp;
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.nblocks) return;
+  const uint64_t stored = ((const GQF_GLOBAL uint64_t*)gqf_block_ptr(f, b))[3];
+  if (stored != used_mask(f, b)) atomicAdd(bad, 1ULL);
 }
 
 extern "C" __global__ void __launch_bounds__(kThreads)
@@ -460,6 +456,12 @@ void gqf_launch_enum_emit(const QF* qf_dev, uint64_t w_lo, uint64_t nwords, unsi
                      starts, (const uint32_t*)local_prefix, (const uint64_t*)group_base, o, hashes, counts, values,
                      cap);
   gqf_prof_end("enum_emit", s);
+}
+
+void gqf_launch_check_used(const QF* qf_dev, uint64_t nblocks, unsigned long long* bad, hipStream_t s) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(gqf_check_used, dim3((uint32_t)((nblocks + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     qf_dev, bad);
 }
 
 void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint32_t* wg_starts, uint64_t* wg_base,

@@ -87,6 +87,12 @@ void gqf_launch_region_lists(const QF* qf_dev, uint64_t nregions, uint64_t n, ui
 #define GQF_REBUILD_INSERT 0
 #define GQF_REBUILD_DELETE 1
 #define GQF_REBUILD_SEQ 2    // items tagged as above, applied in tag order per hash
+// ORed into the mode: plain insert batches take the run-by-run walks
+// instead of the shift path (both give the same table)
+#define GQF_REBUILD_NO_SHIFT 0x100
+// the mode flag the launchers add: GQF_REBUILD_NO_SHIFT when the
+// environment sets GQF_SHIFT_PATH=0 (read once), else 0
+int gqf_rebuild_mode_flags();
 
 // request boundaries of a sequenced call, passed by value to its kernels:
 // request r owns keys [end[r - 1], end[r]) and has tag[r] = seq << 1 | delete
@@ -164,6 +170,9 @@ void gqf_launch_enum_emit(const QF* qf_dev, uint64_t w_lo, uint64_t nwords, unsi
 // workgroup and their bases (nwg + 1, the last = number of clusters);
 // lengths[rank] (rank < cap), hist[min(len, nbins - 1)] += 1 (nbins <= 4096,
 // 0 for none), *max_len = max(*max_len, longest)
+// *bad += number of blocks whose stored used-slot mask disagrees with
+// their header
+void gqf_launch_check_used(const QF* qf_dev, uint64_t nblocks, unsigned long long* bad, hipStream_t s);
 void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint32_t* wg_starts, uint64_t* wg_base,
                        uint64_t* lengths, uint64_t cap, unsigned long long* hist, uint32_t nbins,
                        unsigned long long* max_len, hipStream_t s);

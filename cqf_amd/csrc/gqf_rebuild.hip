@@ -29,6 +29,8 @@
 // wave that owns the next region.  A region whose push is not absorbed in
 // its window is flagged and redone by a single-workgroup pass with a wide
 // window, in ascending region order, after the phase.
+#include <stdlib.h>
+
 #include "gqf_internal.h"
 #include "gqf_launch.h"
 
@@ -60,6 +62,7 @@ struct Smem {
   };
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
+  uint64_t used_out[W];  // shift path: new used-slot masks
   union {
     uint32_t items[kItemCap];  // fast path: hash - first hash of the region
     struct {
@@ -455,6 +458,329 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
 }
 
 // ---------------------------------------------------------------------------
+// Shift path: insert batches whose items are all cached (one item adds one
+// instance).  Inserts only grow runs, so the new layout is the old one with
+// each byte moved right by a shift that
+//   * grows by (new length - old length) at every listed quotient's event
+//     position -- the start of its old run, or for a quotient without one
+//     the slot a new run would take, max(q, end of the runs before it + 1);
+//   * drops by one, down to 0, at every empty slot (an empty slot absorbs
+//     one unit: the run after it is at home, max(q, ...) takes q);
+//   * is unchanged across used slots.
+// Each of these is a map x -> max(a, x + b), and maps compose, so lane j
+// folds slot-block j's events and empty slots (from the block's stored
+// used-slot mask) into one map and a wave scan gives the shift entering
+// every block.  The untouched bytes then move as whole stretches between
+// events and empty slots, instead of run by run; only the listed quotients'
+// runs are re-encoded (merge_quotient).  Offsets come from the new run-end
+// bits by rank: the runs of quotients < 64 j end at the k-th run end of the
+// window, k = runs ending before it + occupied quotients of blocks < j.
+
+// position of a listed quotient q's event: its old run [st, st + ol) if q
+// is occupied, else st = the slot its new run would start at (ol = 0);
+// false if the run ends past the window
+template <int W>
+__device__ __forceinline__ bool locate_run_or_slot(const Smem<W>& sm, int nwin, int32_t q, int32_t& st,
+                                                   uint32_t& ol) {
+  const int j = q >> 6;
+  const int bit = q & 63;
+  const uint64_t occ = sm.occ_in[j];
+  const bool occupied = (occ >> bit) & 1ULL;
+  const int before = __popcll(occ & ((1ULL << bit) - 1ULL));  // block j's occupied quotients below q
+  const int need = occupied ? before + 1 : before;            // run ends to find after xold[j]
+  int32_t prev = sm.xold[j], end = -1;
+  if (need > 0) {
+    const int32_t pos = sm.xold[j] + 1;
+    int bj = pos >> 6;
+    if (bj >= nwin) return false;
+    uint64_t w = sm.run_in[bj] & (~0ULL << (pos & 63));
+    int seen = 0;
+    for (;;) {
+      const int c = __popcll(w);
+      if (before >= 1 && before - 1 >= seen && before - 1 < seen + c) prev = 64 * bj + gqf_select64(w, before - 1 - seen);
+      if (need - 1 < seen + c) {
+        if (occupied) end = 64 * bj + gqf_select64(w, need - 1 - seen);
+        break;
+      }
+      seen += c;
+      if (++bj >= nwin) return false;
+      w = sm.run_in[bj];
+    }
+  }
+  st = q > prev + 1 ? q : prev + 1;
+  ol = occupied ? (uint32_t)(end - st + 1) : 0u;
+  return st < 64 * nwin;
+}
+
+__device__ inline uint64_t range_mask(int lo, int hi) {  // bits [lo, hi) of a word, 0 <= lo <= hi <= 64
+  return gqf_mask((uint64_t)hi) & ~gqf_mask((uint64_t)lo);
+}
+
+// ORs the low `len` bits of `bits` into the window bit array at position d
+// (len <= 64; the bits may straddle two words)
+__device__ inline void or_bits(uint64_t* arr, int32_t d, uint64_t bits, int len) {
+  const int w = d >> 6, sh = d & 63;
+  atomicOr((unsigned long long*)&arr[w], bits << sh);
+  if (sh && sh + len > 64) atomicOr((unsigned long long*)&arr[w + 1], bits >> (64 - sh));
+}
+
+template <int W>
+__device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, int R, int nwin, const Items& src,
+                                                    uint32_t m, Smem<W>& sm, uint64_t occ, uint64_t used,
+                                                    uint64_t touched, bool table_end_in_window) {
+  const int lane = lane_id();
+  const int j = lane;
+  const uint32_t nq = sm.nq;
+  const uint8_t* in = (const uint8_t*)sm.sin;
+  uint8_t* out = (uint8_t*)sm.sout;
+
+  // ---- event position, old length and new length of every listed quotient
+  for (uint32_t t = lane; t < nq; t += 64) {
+    const int32_t q = (int32_t)(sm.items[sm.tit0[t]] >> 8);
+    int32_t st = q;
+    uint32_t ol = 0, nl = 0;
+    if (locate_run_or_slot(sm, nwin, q, st, ol)) {
+      const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
+      nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st, ol, 0, src.hash(sm.items, sm.tit0[t]));
+    } else {
+      atomicMin(&sm.first_invalid, -1);
+      st = 0;
+    }
+    sm.tst[t] = (int16_t)st;
+    sm.tol[t] = (uint16_t)ol;
+    sm.tnl[t] = (uint16_t)nl;
+  }
+  __syncthreads();
+  if (sm.first_invalid < 0) return kNeedWide;
+
+  // ---- the events of slot-block j: listed quotients e0 .. e1 - 1 (their
+  // positions ascend with the quotients)
+  uint32_t e0 = nq;
+  {
+    uint32_t lo = 0, hi = nq;
+    const int32_t key = 64 * j;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sm.tst[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    e0 = lo;
+  }
+  uint32_t e1 = (uint32_t)__shfl_down((int)e0, 1, 64);
+  if (lane == 63) e1 = nq;
+
+  // ---- lane j's shift map: events and (valid) empty slots of slot-block j
+  uint32_t vlim = 0;
+  if (j < nwin) {
+    const uint64_t first = (B0 + (uint64_t)j) * 64;
+    vlim = !table_end_in_window ? 64u : (first >= qf.xnslots ? 0u : (uint32_t)min((uint64_t)64, qf.xnslots - first));
+  }
+  const uint64_t zero_ok = ~used & gqf_mask(vlim);  // empty slots that absorb shift
+  int32_t A = kNegInf, Bm = 0;
+  {
+    int pos = 0;
+    for (uint32_t t = e0; t < e1; t++) {
+      const int rel = sm.tst[t] - 64 * j;
+      const int c = __popcll(zero_ok & range_mask(pos, rel));
+      if (c) compose(A, Bm, 0, -c);
+      compose(A, Bm, kNegInf, (int32_t)sm.tnl[t] - (int32_t)sm.tol[t]);
+      pos = rel;
+    }
+    const int c = __popcll(zero_ok & range_mask(pos, 64));
+    if (c) compose(A, Bm, 0, -c);
+  }
+  // inclusive scan of the maps over the lanes; the shift entering block j
+  // is the exclusive prefix applied to 0
+  int32_t ia = A, ib = Bm;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t pa = __shfl_up(ia, d, 64);
+    const int32_t pb = __shfl_up(ib, d, 64);
+    if (lane >= d) {
+      const int32_t na = pa + ib;
+      ia = na > ia ? na : ia;
+      ib = pb + ib;
+    }
+  }
+  int32_t ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64);
+  if (lane == 0) { ea = kNegInf; eb = 0; }
+  const int32_t delta_in = ea > eb ? ea : eb;  // max(a, 0 + b)
+  const int32_t la = __shfl(ia, 63, 64), lb = __shfl(ib, 63, 64);
+  const int32_t delta_end = la > lb ? la : lb;
+
+  // ---- b*: the first block at or after R, past the last event's old run,
+  // that no shift reaches; blocks [0, b*) are rewritten, the rest stay
+  {
+    int lo = R;
+    if (nq) {
+      const int32_t last = sm.tst[nq - 1] + (sm.tol[nq - 1] ? (int32_t)sm.tol[nq - 1] : 1) - 1;
+      if ((last >> 6) + 1 > lo) lo = (last >> 6) + 1;
+    }
+    const uint64_t bm = __ballot(j >= lo && j < nwin && delta_in == 0);
+    if (lane == 0) {
+      int res = kOk, bstar = nwin;
+      if (bm) {
+        bstar = __builtin_ctzll(bm);
+      } else if (!table_end_in_window) {
+        res = kNeedWide;
+      } else if (delta_end != 0) {
+        res = kNoSpace;
+      }
+      if (sm.xold[0] + 1 > 64 * nwin) res = kNeedWide;  // spill-in longer than the window
+      sm.bstar = bstar;
+      sm.result = res;
+    }
+  }
+  __syncthreads();
+  const int res = sm.result;
+  if (res != kOk) return res;
+  const int bstar = sm.bstar;
+
+  // ---- clear the rows and bit words to rewrite
+  for (uint32_t c = lane; c < (uint32_t)bstar * 17; c += 64) sm.sout[c] = 0;
+  if (j < bstar) {
+    sm.run_out[j] = 0;
+    sm.used_out[j] = 0;
+  }
+  __syncthreads();
+
+  // ---- lane j moves the untouched bytes of slot-block j and records the
+  // new start of each of its events' runs
+  if (j < bstar) {
+    const uint64_t run_j = sm.run_in[j];
+    int32_t delta = delta_in;
+    int32_t cur = 64 * j;
+    if (e0 > 0) {  // an earlier event's old run may reach into this block
+      const int32_t pe = sm.tst[e0 - 1] + (int32_t)sm.tol[e0 - 1];
+      if (pe > cur) cur = pe;
+    }
+    // moves the used stretches of [a, b) (inside block j) right by `delta`;
+    // each valid empty slot on the way lowers delta by one (not below 0)
+    auto stretches = [&](int32_t a, int32_t b) {
+      const int32_t base = 64 * j;
+      if (b > base + 64) b = base + 64;
+      while (a < b) {
+        const int ra = a - base, rb = b - base;
+        const uint64_t nu = ~used & range_mask(ra, rb);
+        const int re = nu ? __builtin_ctzll(nu) : rb;
+        if (re > ra) {
+          const int len = re - ra;
+          const int32_t d = a + delta;
+          if (d + len <= 64 * bstar) {  // always, by the choice of b*; checked so no write leaves the rows
+            for (int k = 0; k < len; k++) out[sin_off(d + k)] = in[sin_off(a + k)];
+            const uint64_t lm = gqf_mask((uint64_t)len);
+            or_bits(sm.run_out, d, (run_j >> ra) & lm, len);
+            or_bits(sm.used_out, d, lm, len);
+          } else {
+            atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+          }
+        }
+        if (re >= rb) break;
+        // the gap of unused slots at re, up to rb
+        const uint64_t rest = ~(~used >> re);
+        int g = rest ? __builtin_ctzll(rest) : 64 - re;
+        if (g > rb - re) g = rb - re;
+        const int gv = re >= (int)vlim ? 0 : ((int)vlim - re < g ? (int)vlim - re : g);
+        delta = delta > gv ? delta - gv : 0;
+        a = base + re + g;
+      }
+    };
+    for (uint32_t t = e0; t < e1; t++) {
+      const int32_t st = sm.tst[t];
+      stretches(cur, st);
+      sm.tsn[t] = (int16_t)(st + delta);
+      delta += (int32_t)sm.tnl[t] - (int32_t)sm.tol[t];
+      const int32_t pe = st + (int32_t)sm.tol[t];
+      if (pe > cur) cur = pe;
+    }
+    stretches(cur, 64 * j + 64);
+  }
+  __syncthreads();
+
+  // ---- the listed quotients' runs at their new starts
+  for (uint32_t t = lane; t < nq; t += 64) {
+    const uint32_t nl = sm.tnl[t];
+    const int32_t sn = sm.tsn[t];
+    const int32_t en = sn + (int32_t)nl - 1;
+    if (sn < 0 || en >= 64 * bstar) {  // cannot happen (see above); no write leaves the rows
+      atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+      continue;
+    }
+    const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
+    merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sn, src.hash(sm.items, sm.tit0[t]));
+    atomicOr((unsigned long long*)&sm.run_out[en >> 6], 1ULL << (en & 63));
+    // the run's slots are used (at most 2 words for runs below 64 slots)
+    for (int32_t p = sn; p <= en;) {
+      const int len = min(64 - (p & 63), en - p + 1);
+      atomicOr((unsigned long long*)&sm.used_out[p >> 6], gqf_mask((uint64_t)len) << (p & 63));
+      p += len;
+    }
+  }
+  __syncthreads();
+
+  // ---- offsets by rank over the new run ends: occupied quotients of the
+  // window's blocks < j, plus the earlier regions' runs that end inside the
+  // window (run ends at or before xold[0])
+  const uint64_t occ_new = j < nwin ? (occ | touched) : 0ULL;
+  const uint64_t re_w = j < bstar ? sm.run_out[j] : (j < nwin ? sm.run_in[j] : 0ULL);
+  uint32_t c_occ = (uint32_t)__popcll(occ_new), c_re = (uint32_t)__popcll(re_w);
+  uint32_t c_pre = 0;
+  {
+    const int32_t x0 = sm.xold[0];
+    if (j < nwin && x0 >= 64 * j) {
+      const int upto = x0 - 64 * j + 1;
+      c_pre = (uint32_t)__popcll(sm.run_in[j] & gqf_mask((uint64_t)(upto < 64 ? upto : 64)));
+    }
+  }
+  uint32_t inc_occ = c_occ, inc_re = c_re, tot_pre = c_pre;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t a1 = (uint32_t)__shfl_up((int)inc_occ, d, 64);
+    const uint32_t a2 = (uint32_t)__shfl_up((int)inc_re, d, 64);
+    if (lane >= d) { inc_occ += a1; inc_re += a2; }
+    tot_pre += (uint32_t)__shfl_xor((int)tot_pre, d, 64);
+  }
+  sm.xnew[j] = (int32_t)(inc_re - c_re);  // run ends in blocks < j
+  if (lane == 63) sm.xnew[64] = (int32_t)inc_re;
+  __syncthreads();
+  uint32_t off_new = 0;
+  if (j < bstar) {
+    const uint32_t k = tot_pre + inc_occ - c_occ;  // runs of quotients < 64 j ending in the window
+    int32_t x = 64 * j - 1;
+    if (k > 0) {
+      if (k > (uint32_t)sm.xnew[nwin]) {
+        x = sm.xold[j];  // ends past the window, where nothing moved
+      } else {
+        int lo = 0, hi = nwin - 1;  // the block holding the k-th run end: cum[b] < k <= cum[b + 1]
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if ((uint32_t)sm.xnew[mid] < k) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t w = lo < bstar ? sm.run_out[lo] : sm.run_in[lo];
+        const int32_t p = 64 * lo + gqf_select64(w, (int)(k - (uint32_t)sm.xnew[lo]) - 1);
+        if (p > x) x = p;
+      }
+    }
+    off_new = (uint32_t)(x - (64 * j - 1));
+  }
+
+  // ---- store blocks [0, b*): lane j writes block j's line
+  if (j < bstar) {
+    GQF_GLOBAL uint4* g = gqf_gptr_mut((uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
+    const uint64_t run = sm.run_out[j];
+    const uint64_t un = sm.used_out[j];
+    const uint32_t* row = sm.sout + 17 * j;
+    GQF_GLOBAL uint4* b = g + (size_t)j * (GQF_BLOCK_BYTES / 16);
+    b[0] = make_uint4((uint32_t)occ_new, (uint32_t)(occ_new >> 32), (uint32_t)run, (uint32_t)(run >> 32));
+    b[1] = make_uint4(off_new, 0, (uint32_t)un, (uint32_t)(un >> 32));
+    b[4] = make_uint4(row[0], row[1], row[2], row[3]);
+    b[5] = make_uint4(row[4], row[5], row[6], row[7]);
+    b[6] = make_uint4(row[8], row[9], row[10], row[11]);
+    b[7] = make_uint4(row[12], row[13], row[14], row[15]);
+  }
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
 // Region rebuild.  All 64 lanes of the (single-wave) workgroup call it.
 // Lane l handles window blocks l, l + 64, ...
 
@@ -473,6 +799,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
                               Smem<W>& sm, uint32_t* lcnt, bool concurrent, uint64_t tag = 0) {
   constexpr int kRounds = W / 64;
+  const bool shift_ok = !(mode & GQF_REBUILD_NO_SHIFT);
+  mode &= ~GQF_REBUILD_NO_SHIFT;
   const int lane = lane_id();
   const int R = (int)qf.region_blocks;
   const uint64_t B0 = r * (uint64_t)R;
@@ -490,9 +818,10 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   // lane j reads block j's line: header into registers, slots into LDS row j
   uint64_t occ_w[kRounds];
   uint32_t off_w[kRounds];
+  uint64_t used_w[kRounds];
   {
     const GQF_GLOBAL uint4* g = gqf_block_ptr(qf, B0);
-    uint4 hv[kRounds], s0[kRounds], s1[kRounds], s2[kRounds], s3[kRounds];
+    uint4 hv[kRounds], h1[kRounds], s0[kRounds], s1[kRounds], s2[kRounds], s3[kRounds];
     uint32_t ov[kRounds];
 #pragma unroll
     for (int k = 0; k < kRounds; k++) {
@@ -500,7 +829,8 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       const int jj = j < nwin ? j : nwin - 1;  // clamped: every lane loads, all in flight together
       const GQF_GLOBAL uint4* b = g + (size_t)jj * (GQF_BLOCK_BYTES / 16);
       hv[k] = b[0];
-      ov[k] = ((const GQF_GLOBAL uint32_t*)b)[4];
+      h1[k] = b[1];  // offset, 0, used-slot mask
+      ov[k] = h1[k].x;
       s0[k] = b[4];
       s1[k] = b[5];
       s2[k] = b[6];
@@ -519,7 +849,9 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
       const int j = lane + 64 * k;
       occ_w[k] = 0;
       off_w[k] = 0;
+      used_w[k] = 0;
       if (j < nwin) {
+        used_w[k] = (uint64_t)h1[k].z | ((uint64_t)h1[k].w << 32);
         const uint64_t occ = (uint64_t)hv[k].x | ((uint64_t)hv[k].y << 32);
         const uint64_t run = (uint64_t)hv[k].z | ((uint64_t)hv[k].w << 32);
         occ_w[k] = occ;
@@ -583,6 +915,10 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
   for (int k = 0; k < kRounds; k++) {
     const int j = lane + 64 * k;
     tm_w[k] = fast && j < nwin ? sm.tmask[j] : 0ULL;
+  }
+  if constexpr (W == kWinFast) {
+    if (shift_ok && fast && kind == kAdd)
+      return rebuild_insert_shift<W>(qf, B0, R, nwin, src, m, sm, occ_w[0], used_w[0], tm_w[0], table_end_in_window);
   }
   if (fast) {
     // old run and new length of every listed quotient, one quotient per lane
@@ -752,10 +1088,12 @@ __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const ui
           occ = occ_new[k];
         }
         const uint64_t run = sm.run_out[j];
+        const uint64_t lim = qf.xnslots - (B0 + (uint64_t)j) * 64;
+        const uint64_t used = gqf_used_mask(occ, run, off, lim < 64 ? (uint32_t)lim : 64u);
         const uint32_t* row = sm.sout + 17 * j;
         GQF_GLOBAL uint4* b = g + (size_t)j * (GQF_BLOCK_BYTES / 16);
         b[0] = make_uint4((uint32_t)occ, (uint32_t)(occ >> 32), (uint32_t)run, (uint32_t)(run >> 32));
-        b[1] = make_uint4(off, 0, 0, 0);
+        b[1] = make_uint4(off, 0, (uint32_t)used, (uint32_t)(used >> 32));
         b[4] = make_uint4(row[0], row[1], row[2], row[3]);
         b[5] = make_uint4(row[4], row[5], row[6], row[7]);
         b[6] = make_uint4(row[8], row[9], row[10], row[11]);
@@ -1170,6 +1508,15 @@ gqf_group_wide(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
 // ---------------------------------------------------------------------------
 // launchers
 
+int gqf_rebuild_mode_flags() {
+  static int flags = -1;
+  if (flags < 0) {
+    const char* v = getenv("GQF_SHIFT_PATH");
+    flags = v && v[0] == '0' ? GQF_REBUILD_NO_SHIFT : 0;
+  }
+  return flags;
+}
+
 void gqf_launch_region_bounds(const QF* qf_dev, uint64_t nregions, const uint64_t* items, uint64_t n,
                               const uint64_t* n_dev, uint32_t* region_start, uint32_t* region_list,
                               uint32_t* list_count, uint64_t hmask, hipStream_t s) {
@@ -1203,9 +1550,10 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
   const uint64_t per_parity = (nregions + 1) / 2;
   const uint32_t g = (uint32_t)(n_items < per_parity ? (n_items ? n_items : 1) : per_parity);
   const uint64_t stride = per_parity + 1;
+  mode |= gqf_rebuild_mode_flags();
   for (int parity = 0; parity < 2; parity++) {
     gqf_prof_begin("rebuild_phase", s);
-    if (counts || mode == kSeq)
+    if (counts || (mode & ~GQF_REBUILD_NO_SHIFT) == kSeq)
       hipLaunchKernelGGL(gqf_rebuild_phase_counts, dim3(g), dim3(64), 0, s, qf_dev, items, counts, region_start,
                          region_list + (size_t)parity * stride, list_count, parity, mode, wide_flags, wide_count);
     else
@@ -1250,6 +1598,7 @@ uint32_t gqf_group_rebuild_grid(uint32_t per_cu) {
 
 void gqf_launch_group_rebuild(const gqf_group_dev& g, const uint64_t* items, int mode, uint32_t grid,
                               hipStream_t s) {
+  mode |= gqf_rebuild_mode_flags();
   gqf_prof_begin("group_rebuild", s);
   hipLaunchKernelGGL(gqf_group_rebuild, dim3(grid ? grid : 1), dim3(64), 0, s, g, items, mode);
   gqf_prof_end("group_rebuild", s);

@@ -40,7 +40,9 @@ EXPORTS = [
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
     "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu", "gqf_group_set_wait_bound",
-    "qf_batch_pieces",
+    "qf_batch_pieces", "qf_check_layout",
+    "qf_server_create", "qf_server_add_filter", "qf_server_submit", "qf_server_run", "qf_server_wait",
+    "qf_server_request_times", "qf_server_get_stats", "qf_server_filter", "qf_server_destroy",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
     "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
     "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust",
@@ -132,6 +134,8 @@ def load_library(path=LIB_PATH):
     L.gqf_group_get_workgroups_per_cu.restype = ctypes.c_int
     L.gqf_group_set_wait_bound.argtypes = [_vp, ctypes.c_uint32]
     L.gqf_group_set_wait_bound.restype = ctypes.c_int
+    L.qf_check_layout.argtypes = [_vp, ctypes.POINTER(_u64)]
+    L.qf_check_layout.restype = ctypes.c_int
     L.qf_batch_pieces.argtypes = [_u64, _u64, ctypes.POINTER(_u64), ctypes.c_uint32]
     L.qf_batch_pieces.restype = ctypes.c_uint32
     L.qf_malloc_device_kv.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
@@ -399,6 +403,15 @@ class GQF:
 
     def clear_status(self):
         self._lib.qf_clear_status(self.handle)
+
+    def layout_errors(self):
+        """Blocks whose stored used-slot mask disagrees with their runs
+        (0 for a consistent table; synchronises)."""
+        bad = _u64()
+        rc = self._lib.qf_check_layout(self.handle, ctypes.byref(bad))
+        if rc != 0:
+            raise RuntimeError(f"qf_check_layout rc={rc}")
+        return int(bad.value)
 
     def export_image(self):
         buf = np.empty(self.nbytes, dtype=np.uint8)

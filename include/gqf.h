@@ -205,6 +205,52 @@ int gqf_group_set_wait_bound(gqf_group *g, uint32_t max_polls);
 int gqf_group_set_workgroups_per_cu(gqf_group *g, uint32_t per_cu);
 int gqf_group_get_workgroups_per_cu(gqf_group *g, uint32_t *per_cu);
 
+/* ---- serving loop (no reference counterpart) -----------------------
+ * Many resident filters, requests of any size and kind arriving over time,
+ * served by continuous batching on a native thread: each round, every
+ * filter with queued requests and fewer than max_inflight batches on the
+ * device takes the head of its queue -- gets, inserts and deletes in
+ * arrival order, up to max_batch_keys keys and QF_SEQ_MAX_REQUESTS requests
+ * per kind -- as one sequenced batch (bulk_get_async, qf_bulk_mutate_seq,
+ * qf_bulk_get_seq_adjust) on the filter's stream, filters spread over
+ * n_streams streams.  Each get returns the counts after exactly the
+ * requests that arrived before it, as if the requests ran one at a time.
+ * A filter is created on its first request if its device memory fits
+ * budget_bytes; otherwise least recently used idle filters are evicted to
+ * host images (qf_export_image) until it fits, and re-admitted by import.
+ * plain != 0 makes the baseline instead: every request its own bulk call,
+ * in arrival order, on one stream, every filter resident.
+ *
+ * Use: create, add the filters, submit every request with its arrival time
+ * (ns after qf_server_run; times must not decrease), run, wait.  keys is a
+ * device array of n <= max_batch_keys keys that stays valid and unchanged
+ * until the request is done; a get's counts is a device array of n counts.
+ * Returns 0 / a request id, or < 0 for bad arguments. */
+#define QF_SERVER_GET 0
+#define QF_SERVER_INSERT 1
+#define QF_SERVER_DELETE 2
+typedef struct qf_server qf_server;
+typedef struct qf_server_stats {
+  uint64_t batches, evictions, admissions;
+} qf_server_stats;
+int qf_server_create(qf_server **s, uint64_t budget_bytes, uint32_t n_streams, uint64_t max_batch_keys,
+                     uint32_t max_inflight, int plain);
+int qf_server_add_filter(qf_server *s, uint32_t fid, int nbits);
+int64_t qf_server_submit(qf_server *s, uint64_t t_arrival_ns, uint32_t fid, int op, const uint64_t *keys,
+                         uint64_t n, uint64_t *counts);
+/* starts the loop thread; the arrival clock starts here */
+int qf_server_run(qf_server *s);
+/* blocks until every submitted request is done; *wall_ns = completion time
+ * of the last one (ns after qf_server_run) */
+int qf_server_wait(qf_server *s, uint64_t *wall_ns);
+/* times of one request (ns after qf_server_run); returns 1 if it is done */
+int qf_server_request_times(qf_server *s, int64_t rid, uint64_t *t_arrival_ns, uint64_t *t_launch_ns,
+                            uint64_t *t_done_ns);
+int qf_server_get_stats(qf_server *s, qf_server_stats *out);
+/* the resident handle of a filter (NULL if evicted or unknown; not while running) */
+QF *qf_server_filter(qf_server *s, uint32_t fid);
+void qf_server_destroy(qf_server *s);
+
 /* ---- whole-table operations ------------------------------------- */
 
 /* qf_malloc (src/gqf.cu:2073-2105) with value bits: 2^nbits slots of 8
@@ -281,6 +327,12 @@ int64_t qf_multi_merge_device(QF *const *qfs, int nqf, QF *qfr);
  * then left holding its own ascending hashes (the reference sorts the whole
  * array, src/gqf.cu:3209).  Pure host arithmetic. */
 uint32_t qf_batch_pieces(uint64_t n, uint64_t max_piece, uint64_t *starts, uint32_t cap);
+
+/* Consistency check of the device table (no reference counterpart; for
+ * tests): *bad_blocks = number of blocks whose stored used-slot mask (an
+ * internal field of the device layout) disagrees with their runs.
+ * Synchronises; returns 0 or < 0 on error. */
+int qf_check_layout(QF *qf, uint64_t *bad_blocks);
 
 /* last HIP error seen by the library (0 = none); clears it */
 int qf_last_error(void);

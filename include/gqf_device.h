@@ -35,7 +35,13 @@
  *   bytes 17..80 slots[64]  u8 remainders / counter digits
  * On the device each block occupies one 128-byte line, every field aligned:
  *   bytes 0..7   occupieds   bytes 8..15 runends   bytes 16..19 offset (u32)
- *   bytes 20..63 zero        bytes 64..127 slots[64]
+ *   bytes 20..23 zero        bytes 24..31 used     bytes 32..63 zero
+ *   bytes 64..127 slots[64]
+ * `used` has bit i set when slot 64b+i lies inside some run (the slots
+ * offset_lower_bound reports non-empty, src/gqf.cu:681-705).  It is
+ * redundant -- gqf_used_mask derives it from the block's own header -- and
+ * every writer keeps it, so the bulk rebuild finds a block's empty slots
+ * without walking its runs.
  * A probe loads the whole line in one round of back-to-back vector loads:
  * header and, for ~94% of runs at 90% load, the run itself.  Random access
  * on MI355X is bound by the number of dependent line requests, not bytes
@@ -215,6 +221,28 @@ __device__ static inline uint64_t gqf_load_runends(const quotient_filter& qf, ui
   return ((const GQF_GLOBAL uint64_t*)gqf_block_ptr(qf, b))[1];
 }
 
+// The used-slot mask of a block from its header alone: the first `offset`
+// slots hold runs of earlier blocks' quotients; past them, slot i is inside
+// a run iff more of the block's quotients <= i have runs than runs have
+// ended in [offset, i) (each run covers its start .. end without gaps).
+// `lim` = slots of the block below xnslots (64 for every block but the last).
+__host__ __device__ static inline uint64_t gqf_used_mask(uint64_t occ, uint64_t runends, uint32_t offset, uint32_t lim) {
+  uint64_t used;
+  if (offset >= 64) {
+    used = ~0ULL;
+  } else {
+    used = gqf_mask(offset);
+    int p = __builtin_popcountll(occ & gqf_mask(offset));
+    int r = 0;
+    for (uint32_t i = offset; i < 64; i++) {
+      p += (int)((occ >> i) & 1);
+      if (p > r) used |= 1ULL << i;
+      r += (int)((runends >> i) & 1);
+    }
+  }
+  return lim < 64 ? used & gqf_mask(lim) : used;
+}
+
 // byte address of slot s inside the block array
 __host__ __device__ static inline uint64_t gqf_slot_addr(uint64_t s) {
   return (s >> 6) * GQF_BLOCK_BYTES + GQF_SLOT_BASE + (s & 63);
@@ -377,6 +405,11 @@ __host__ __device__ static inline void set_occupied(const QF& f, uint64_t q) {
   uint64_t* w = (uint64_t*)blk(f, q >> 6);
   *w |= 1ULL << (q & 63);
 }
+// marks slot s as inside a run (the used-slot mask at bytes 24..31)
+__host__ __device__ static inline void set_used(const QF& f, uint64_t s) {
+  uint64_t* w = (uint64_t*)blk(f, s >> 6) + 3;
+  *w |= 1ULL << (s & 63);
+}
 
 // end of the runs of quotients < 64 b (at least 64 b - 1)
 __host__ __device__ static inline int64_t end_before_block(const QF& f, uint64_t b) {
@@ -428,8 +461,10 @@ __host__ __device__ static inline uint64_t first_empty(const QF& f, uint64_t fro
 // Opens d free slots at p: the occupied stretch [p, e1) moves right by d,
 // (e1, e2) by d - 1, ..., where e1 < e2 < ... are the first d empty slots at
 // or after p; run-end bits move with their slots.  Slots [p, p + d) are then
-// free (stale bytes and bits).  Returns the last empty slot used, or -1 if
-// fewer than d empty slots exist below xnslots (nothing is moved then).
+// free (stale bytes and bits) for the caller to fill; the d empty slots
+// become used, and the used-slot masks say so.  Returns the last empty slot
+// used, or -1 if fewer than d empty slots exist below xnslots (nothing is
+// moved then).
 __host__ __device__ static inline int64_t open_slots(const QF& f, uint64_t p, int d) {
   uint64_t e[4];
   uint64_t from = p;
@@ -445,6 +480,7 @@ __host__ __device__ static inline int64_t open_slots(const QF& f, uint64_t p, in
       set_slot(f, s + sh, slot(f, s));
       set_runend(f, s + sh, is_runend(f, s));
     }
+    set_used(f, e[j]);
   }
   return (int64_t)e[d - 1];
 }

@@ -65,3 +65,23 @@ extern "C" void ph_export(const uint8_t* blocks, int nbits, uint8_t* out) {
     memcpy(dst + 17, src + GQF_SLOT_BASE, 64);
   }
 }
+
+// blocks whose stored used-slot mask (bytes 24..31) disagrees with the one
+// their header implies
+extern "C" uint64_t ph_used_mismatches(const uint8_t* blocks, int nbits) {
+  uint32_t st = 0;
+  const QF f = make_qf(nullptr, nbits, &st);
+  uint64_t bad = 0;
+  for (uint64_t b = 0; b < f.nblocks; b++) {
+    const uint8_t* src = blocks + b * GQF_BLOCK_BYTES;
+    uint64_t occ, run, used;
+    uint32_t off;
+    memcpy(&occ, src, 8);
+    memcpy(&run, src + 8, 8);
+    memcpy(&off, src + 16, 4);
+    memcpy(&used, src + 24, 8);
+    const uint64_t lim = f.xnslots - b * 64;
+    if (used != gqf_used_mask(occ, run, off, lim < 64 ? (uint32_t)lim : 64u)) bad++;
+  }
+  return bad;
+}

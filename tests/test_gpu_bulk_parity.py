@@ -46,7 +46,7 @@ def test_bulk_insert_image_and_counts(gpu, oracle, nbits, load, seed):
         # keys are left as the ascending hashes (src/gqf.cu:3206-3209)
         assert np.array_equal(to_numpy_u64(dk), hk)
         _assert_images_equal(g.export_image(), o.image(), "image after bulk_insert")
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         # counts of the inserted keys and of disjoint keys
         q = np.concatenate([keys, oracle.splitmix64(seed + 100, n)])
         got = to_numpy_u64(g.bulk_get(to_device_u64(q)))
@@ -213,7 +213,7 @@ def test_tuning_does_not_change_results(gpu, oracle, region_blocks, digit):
         _assert_images_equal(g.export_image(), o.image(), "tuned delete")
         got = to_numpy_u64(g.bulk_get(to_device_u64(keys)))
         assert np.array_equal(got, o.bulk_get(keys))
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         o.close()
@@ -242,7 +242,7 @@ def test_wide_window_fallback_at_high_load(gpu, oracle, nbits):
         _assert_images_equal(g.export_image(), o.image(), "delete half")
         got = to_numpy_u64(g.bulk_get(to_device_u64(keys)))
         assert np.array_equal(got, o.bulk_get(keys))
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         o.close()
@@ -274,7 +274,7 @@ def test_sparse_batches_after_dense_fill(gpu, oracle, nbits):
             o.bulk_insert(part.copy())
         gpu.cuda.synchronize()
         _assert_images_equal(g.export_image(), o.image(), "image after sparse batches")
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         o.close()
@@ -300,7 +300,7 @@ def test_large_batches_two_level_sort(gpu, oracle, nbits, sizes):
             o.bulk_insert(hk)
             assert np.array_equal(to_numpy_u64(dk), hk)
         _assert_images_equal(g.export_image(), o.image(), "image after two-level sorted batches")
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         q = oracle.splitmix64(41, 1 << 16)
         assert np.array_equal(to_numpy_u64(g.bulk_get(to_device_u64(q))), o.bulk_get(q))
     finally:

@@ -65,7 +65,7 @@ def test_point_bulk_insert_kernel_and_user_kernel(dropin, oracle):
         assert np.array_equal(f.export_image(), o.image())
         allk = np.concatenate([k1, k2])
         assert dropin.dropin_point_bulk_get(to_device_u64(allk).data_ptr(), allk.size) == 0
-        assert f.status() == 0
+        assert f.status() == 0 and f.layout_errors() == 0
         o.close()
     finally:
         f.handle = None  # the caller destroys its own filter

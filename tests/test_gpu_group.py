@@ -49,7 +49,7 @@ def test_group_insert_matches_oracle(gpu, oracle):
             assert np.array_equal(filters[f].export_image(), o.image()), f
             assert np.array_equal(got_keys[offsets[f]:offsets[f + 1]], seg), f
             assert np.array_equal(rets[offsets[f]:offsets[f + 1]], o.bulk_get(allk[offsets[f]:offsets[f + 1]])), f
-            assert filters[f].status() == 0
+            assert filters[f].status() == 0 and filters[f].layout_errors() == 0
             o.close()
     finally:
         _close(filters, group)
@@ -145,7 +145,7 @@ def test_group_exact_counts_larger(gpu):
             masked = seg & ((1 << (q + 8)) - 1)
             _, inv, cnt = torch.unique(masked, return_inverse=True, return_counts=True)
             assert torch.equal(got[offsets[f]:offsets[f + 1]], cnt[inv]), f
-            assert filters[f].status() == 0
+            assert filters[f].status() == 0 and filters[f].layout_errors() == 0
     finally:
         _close(filters, group)
 
@@ -176,7 +176,8 @@ def test_group_workgroups_per_cu_keep_results(gpu, oracle):
     """The persistent rebuild's grid (workgroups per CU) changes which
     workgroup takes which region, never the result: 1 per CU (each
     workgroup loops over many regions), 3, and the most the GPU keeps
-    resident, replayed from graphs, against the oracle."""
+    resident (found by asking for more), replayed from graphs, against the
+    oracle."""
     from cqf_amd.gqf import GQFGroup
     params, keys, offsets, filters = _setup(10, 12, 15, seed=21)
     _say(f"case group workgroups per CU 10 filters keys={keys.numel()}")
@@ -193,7 +194,16 @@ def test_group_workgroups_per_cu_keep_results(gpu, oracle):
             want.append(o.image())
             o.close()
         dk = torch.empty_like(keys, device="cuda")
-        for pc in (1, 3, 12):
+        most = 0  # the most workgroups per CU the GPU keeps resident (the setter refuses more)
+        for pc in range(16, 0, -1):
+            try:
+                group.workgroups_per_cu = pc
+                most = pc
+                break
+            except ValueError:
+                pass
+        assert most >= 8
+        for pc in (1, 3, most):
             group.workgroups_per_cu = pc
             assert group.workgroups_per_cu <= pc
             for f in filters:
@@ -203,7 +213,7 @@ def test_group_workgroups_per_cu_keep_results(gpu, oracle):
             torch.cuda.synchronize()
             for f in range(len(filters)):
                 assert np.array_equal(filters[f].export_image(), want[f]), (pc, f)
-                assert filters[f].status() == 0
+                assert filters[f].status() == 0 and filters[f].layout_errors() == 0
     finally:
         _close(filters, group)
 
@@ -232,7 +242,7 @@ def test_group_segment_sort_limits(gpu, oracle, big):
             o.bulk_insert(seg)
             assert np.array_equal(filters[f].export_image(), o.image()), f
             assert np.array_equal(got[offsets[f]:offsets[f + 1]], seg), f
-            assert filters[f].status() == 0
+            assert filters[f].status() == 0 and filters[f].layout_errors() == 0
             o.close()
     finally:
         _close(filters, group)

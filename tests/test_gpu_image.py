@@ -50,7 +50,7 @@ def test_import_saturated_offsets_then_query_and_insert(gpu, oracle):
         o.bulk_insert(keys.copy())
         gpu.cuda.synchronize()
         assert np.array_equal(g.export_image(), o.image())
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         o.close()

@@ -36,7 +36,7 @@ def test_point_bulk_insert_matches_bulk_image(gpu, oracle, nbits, load, seed):
         g.point_bulk_insert(dk)
         o.bulk_insert(keys.copy())
         assert np.array_equal(g.export_image(), o.image())
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         other = oracle.splitmix64(seed + 50, n)
         assert g.point_get_misses(to_device_u64(keys)) == 0
         assert g.point_get_misses(to_device_u64(other)) == o.bulk_get_misses(other)

@@ -33,14 +33,14 @@ def test_small_regions_plain_insert_high_load(gpu, oracle, region):
         o.bulk_insert(keys.copy())
         torch.cuda.synchronize()
         assert np.array_equal(f.export_image(), o.image())
-        assert f.status() == 0
+        assert f.status() == 0 and f.layout_errors() == 0
         # and a delete of a third, with the same tuning
         dele = keys[::3].copy()
         f.bulk_delete(to_device_u64(dele))
         o.bulk_delete(dele.copy())
         torch.cuda.synchronize()
         assert np.array_equal(f.export_image(), o.image())
-        assert f.status() == 0
+        assert f.status() == 0 and f.layout_errors() == 0
     finally:
         f.close()
         o.close()
@@ -65,7 +65,7 @@ def test_small_regions_group_insert_high_load(gpu, oracle, region):
             o = oracle.OracleQF(q)
             o.bulk_insert(segs[i].copy())
             assert np.array_equal(filters[i].export_image(), o.image()), i
-            assert filters[i].status() == 0
+            assert filters[i].status() == 0 and filters[i].layout_errors() == 0
             o.close()
     finally:
         group.close()
@@ -100,7 +100,7 @@ def test_group_wait_bound_zero_sends_odd_regions_to_wide_pass(gpu, oracle):
             torch.cuda.synchronize()
             for i in range(len(qs)):
                 assert np.array_equal(filters[i].export_image(), want[i]), i
-                assert filters[i].status() == 0
+                assert filters[i].status() == 0 and filters[i].layout_errors() == 0
             for f in filters:
                 f.reset()
             group.set_wait_bound(1 << 22)

@@ -75,6 +75,6 @@ def test_delete_with_offsets_above_255_keeps_canonical_image(gpu, oracle):
         f.bulk_delete(to_device_u64(rest), flags=KEY_IS_HASH)
         torch.cuda.synchronize()
         assert not f.export_image().any()
-        assert f.status() == 0
+        assert f.status() == 0 and f.layout_errors() == 0
     finally:
         f.close()

@@ -100,7 +100,7 @@ def test_seq_batch_equals_one_by_one(gpu, oracle, nbits, prefill, nreq, max_size
             o.bulk_insert(pre.copy())
         got = _run_seq(g, reqs, gpu)
         want = _run_oracle(o, reqs)
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         img_g, img_o = g.export_image(), o.image()
         assert np.array_equal(img_g, img_o), f"{np.count_nonzero(img_g != img_o)} image bytes differ"
         for i in want:
@@ -162,7 +162,7 @@ def test_seq_batch_with_value_bits(gpu, oracle):
     try:
         assert g.info["value_bits"] == vb
         got = _run_seq(g, reqs, gpu)
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         for i in want:
             assert np.array_equal(got[i], want[i]), f"request {i}"
         live = {h: c for h, c in counts.items() if c}

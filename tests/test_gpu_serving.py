@@ -59,9 +59,7 @@ def test_served_results_equal_alone(gpu, budget_filters):
             assert np.array_equal(got, alone[r.rid]), r.rid
             hits += int((got > 0).sum())
     assert hits > 0
-    for f in srv.filters.values():
-        if f.q is not None:
-            assert f.q.status() == 0
+    assert not any(S.statuses(srv).values())  # status words and used-slot masks of the resident filters
     srv.close()
     # and the plain per-request path serves the same results
     reqs2 = _requests(spec)

@@ -31,7 +31,7 @@ def _filled(oracle, nbits, load, seed, dup_every=0):
         keys = np.concatenate([keys, extra])
     g = GQF(nbits)
     g.bulk_insert(to_device_u64(keys))
-    assert g.status() == 0, "the test's keys overflow the table"
+    assert g.status() == 0 and g.layout_errors() == 0, "the test's keys overflow the table"
     return g, keys
 
 
@@ -80,7 +80,7 @@ def test_enumerate_keys_recover_small_keys(gpu, oracle):
     g = GQF(nbits)
     try:
         g.bulk_insert(to_device_u64(np.repeat(keys, counts)))
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
         k, c = g.enumerate(form=QF_ENUM_KEY)
         got = dict(zip(to_numpy_u64(k).tolist(), to_numpy_u64(c).tolist()))
         assert got == dict(zip(keys.tolist(), counts.tolist()))
@@ -269,7 +269,7 @@ def test_resize_keeps_counts_and_layout(gpu, oracle, nbits, grow, dup):
         g.bulk_delete(to_device_u64(allk))
         gpu.cuda.synchronize()
         assert not g.export_image().any()
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         if k is not None:

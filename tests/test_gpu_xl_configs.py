@@ -81,7 +81,7 @@ def test_config2_insert_image_and_sorted_keys(cfg2, oracle):
     f, o = cfg2["f"], cfg2["o"]
     _say("  config 2: image after the inserts")
     assert np.array_equal(f.export_image(), o.image())
-    assert f.status() == 0
+    assert f.status() == 0 and f.layout_errors() == 0
     # each batch's keys are left as its ascending hashes (first and last batch)
     mask = (1 << (Q2 + 8)) - 1
     b = cfg2["bounds"]
@@ -139,7 +139,7 @@ def test_config2_half_then_full_delete(cfg2):
     torch.cuda.synchronize()
     assert not f.export_image().any()
     assert f.bulk_get_misses(master) == N2  # src/gqf_verify.cu:439
-    assert f.status() == 0
+    assert f.status() == 0 and f.layout_errors() == 0
 
 
 def test_config3_zipf_exact_multiplicities(gpu, oracle):
@@ -155,7 +155,7 @@ def test_config3_zipf_exact_multiplicities(gpu, oracle):
         f.bulk_insert_reduce(draws)
         got = f.bulk_get(keys)
         assert torch.equal(got, mult)
-        assert f.status() == 0
+        assert f.status() == 0 and f.layout_errors() == 0
         _say(f"  exact multiplicities ({time.time() - t0:.1f} s); canonical image")
         mask = (1 << (q + 8)) - 1
         hk = oracle.hash_many(keys.cpu().numpy().view(np.uint64), mask)
@@ -193,7 +193,7 @@ def test_config4_256_kmer_filters(gpu, oracle):
             masked = keys[a:b] & ((1 << (q + 8)) - 1)
             _, inv, cnt = torch.unique(masked, return_inverse=True, return_counts=True)
             assert torch.equal(got[a:b], cnt[inv]), fi
-            assert filters[fi].status() == 0, fi
+            assert filters[fi].status() == 0 and filters[fi].layout_errors() == 0, fi
             # the segment holds its ascending hashes: their run lengths are
             # the multiset, whose canonical image the filter must hold
             h, c = torch.unique_consecutive(dk[a:b], return_counts=True)

@@ -30,7 +30,7 @@ def test_zipf_reduce_exact_and_image(gpu, oracle, batches):
         assert np.array_equal(got, mult.numpy())
         assert int(mult[0]) > 100000  # multi-digit counters in play
         assert np.array_equal(g.export_image(), o.image())
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
         o.close()
@@ -47,7 +47,7 @@ def test_zipf_reduce_exact_q24(gpu):
         g.bulk_insert_reduce(draws.clone())
         got = g.bulk_get(keys)
         assert torch.equal(got, mult)
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         g.close()
 
@@ -84,7 +84,7 @@ def test_reduce_counting_path_matches_oracle(gpu, oracle):
             assert np.array_equal(g.export_image(), o.image()), f"image after batch {rep}"
         got = to_numpy_u64(g.bulk_get(keys.cuda())).astype(np.int64)
         assert np.array_equal(got, 2 * mult.numpy())
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         gqf.profile_enable(False)
         g.close()
@@ -128,7 +128,7 @@ def test_reduce_counting_falls_back_when_table_overflows(gpu, oracle):
         o.bulk_insert_reduce(ok)
         assert np.array_equal(to_numpy_u64(dk), ok)
         assert np.array_equal(g.export_image(), o.image())
-        assert g.status() == 0
+        assert g.status() == 0 and g.layout_errors() == 0
     finally:
         gqf.profile_enable(False)
         g.close()

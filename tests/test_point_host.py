@@ -30,6 +30,8 @@ def ph():
     L.ph_count.argtypes = [vp, ctypes.c_int, u64]
     L.ph_count.restype = u64
     L.ph_export.argtypes = [vp, ctypes.c_int, vp]
+    L.ph_used_mismatches.argtypes = [vp, ctypes.c_int]
+    L.ph_used_mismatches.restype = u64
     return L
 
 
@@ -54,6 +56,8 @@ class HostTable:
         return int(self.ph.ph_count(self._p(), self.nbits, int(h)))
 
     def image(self):
+        # every image read also checks the device layout's used-slot masks
+        assert self.ph.ph_used_mismatches(self._p(), self.nbits) == 0
         out = np.zeros(self.nblocks * 81, dtype=np.uint8)
         self.ph.ph_export(self._p(), self.nbits, out.ctypes.data_as(ctypes.c_void_p))
         return out
