@@ -41,19 +41,21 @@ def parse():
     ap.add_argument("--batch-log2", type=int, default=24)
     ap.add_argument("--load", type=float, default=0.9)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-nbits", type=int, default=26,
-                    help="table size of the CPU baseline sample (2^26: about 25 s on one core)")
+    ap.add_argument("--cpu-nbits", type=int, default=28,
+                    help="table size of the CPU baseline (2^28, the config-2 table: about 15 s on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=None,
                     help="rocprofv3 counter summary holding the measured HBM bytes per launch "
                          "(default: profiles/traffic_q28.md, profiles/traffic_table_q28.md for --workload table)")
-    ap.add_argument("--workload", choices=["bulk", "zipf", "groups", "serve", "table"], default="bulk",
+    ap.add_argument("--workload", choices=["bulk", "zipf", "groups", "serve", "table", "loadcurve"], default="bulk",
                     help="bulk: uniform insert/get/delete (default); zipf: skewed bulk_insert_reduce; "
                          "groups: 256 k-mer filters in one persistent launch; serve: served request stream; "
-                         "table: enumeration, census, statistics, file image and growth of a full table")
+                         "table: enumeration, census, statistics, file image and growth of a full table; "
+                         "loadcurve: insert and lookup rates at every 5% load step to 95%")
     ap.add_argument("--serve-requests", type=int, default=6000)
     ap.add_argument("--serve-rate", type=float, default=2.0e5, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-budget-frac", type=float, default=0.5,
+                    help="HBM budget of the served filters as a fraction of their total size (evictions below 1)")
     ap.add_argument("--serve-streams", type=int, default=8)
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--groups-filters", type=int, default=256)
@@ -93,34 +95,162 @@ def read_traffic(path, kernel):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (platform.processor()
+    is empty on the GPU hosts)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.machine()
+
+
+def cpu_config1():
+    """Config 1 exactly: the oracle (reference host path restated in C), one
+    core: a 2^20-slot table, 943,718 inserts of splitmix64(1) keys, queries
+    of those keys and of 2^20 splitmix64(2) keys; median of 5 runs."""
+    from oracle import oracle as O
+    import numpy as np
+    nbits, n = 20, 943_718
+    keys = O.splitmix64(1, n)
+    other = O.splitmix64(2, 1 << 20)
+    runs = []
+    for _ in range(5):
+        o = O.OracleQF(nbits)
+        t_ins = o.time_inserts(keys)
+        t_in, hits_in = o.time_queries(keys)
+        t_out, fp = o.time_queries(other)
+        o.close()
+        runs.append((t_ins, t_in, t_out, hits_in, fp))
+    runs.sort(key=lambda r: r[0] + r[1] + r[2])
+    t_ins, t_in, t_out, hits_in, fp = runs[len(runs) // 2]
+    return {"workload": "config 1: 2^20 slots, 943,718 splitmix64(1) inserts, then those keys and 2^20 "
+                        "splitmix64(2) keys queried (median of 5 runs, one core)",
+            "inserts_mops": round(n / t_ins / 1e6, 3), "queries_present_mops": round(n / t_in / 1e6, 3),
+            "queries_disjoint_mops": round(other.size / t_out / 1e6, 3),
+            "present_found": hits_in, "disjoint_hits": fp, "fp_rate": round(fp / other.size, 6)}
+
+
 def cpu_baseline(nbits, seed):
     """The oracle (the reference host path, qf_insert / qf_count_key_value,
-    restated in C) on a bounded sample of the same workload: a 2^nbits-slot
-    table filled to 90% load with splitmix64(seed) keys one qf_insert at a
-    time, then the inserted keys and 2^nbits disjoint keys (the next outputs
-    of the same stream) queried; one core."""
+    restated in C; one core) on a bounded sample of the config-2 workload
+    itself: the same 2^nbits-slot table and splitmix64(seed) key stream.
+    Inserts are timed in each tenth of the fill to 90% load: the slice is
+    loaded untimed (the oracle's bulk insert, whose table is the same), then
+    its last 2^21 keys are timed as sequential qf_insert calls and the
+    slice's time scaled from them.  Queries are timed at 90% load on 2^22
+    inserted and 2^22 disjoint keys and scaled to the workload's 2^nbits
+    disjoint + all inserted keys."""
     from oracle import oracle as O
     import numpy as np
     n = int(0.9 * (1 << nbits))
     keys = O.splitmix64(seed, n)
-    q = O.splitmix64(seed, 1 << nbits, skip=n)
-    qk = np.concatenate([keys, q])
-    t_ins, t_q, hits = O.time_insert_query(nbits, keys, qk)
-    ops = n + qk.size
+    sample = 1 << 21
+    o = O.OracleQF(nbits)
+    t_ins_est, timed_ins, t_timed = 0.0, 0, 0.0
+    bounds = [n * k // 9 for k in range(10)]
+    for a, b in zip(bounds, bounds[1:]):
+        cut = max(a, b - sample)
+        o.bulk_insert(keys[a:cut].copy())
+        t = o.time_inserts(keys[cut:b])
+        t_timed += t
+        timed_ins += b - cut
+        t_ins_est += t * (b - a) / (b - cut)
+    qs = 1 << 22
+    t_p, hits = o.time_queries(keys[:: max(1, n // qs)][:qs])
+    other = O.splitmix64(seed, qs, skip=n)
+    t_d, _fp = o.time_queries(other)
+    o.close()
+    n_q = n + (1 << nbits)
+    t_q_est = (t_p / qs) * n + (t_d / qs) * (1 << nbits)
     return {
-        "value": round(ops / (t_ins + t_q) / 1e6, 3),
+        "value": round((n + n_q) / (t_ins_est + t_q_est) / 1e6, 3),
         "unit": "Mops/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"the same workload on a 2^{nbits}-slot table: {n} sequential inserts of "
-                   f"splitmix64(seed={seed}) keys to 90% load, then {qk.size} queries (inserted + "
-                   f"2^{nbits} disjoint); reference host path restated in C, single thread"),
-        "inserts_mops": round(n / t_ins / 1e6, 3),
-        "queries_mops": round(qk.size / t_q / 1e6, 3),
-        "seconds": round(t_ins + t_q, 3),
-        "cpu_model": platform.processor() or platform.machine(),
+        "sample": (f"the config-2 workload on its own 2^{nbits}-slot table and splitmix64(seed={seed}) keys: "
+                   f"{timed_ins} sequential qf_insert calls timed, the last 2^21 of each tenth of the fill "
+                   f"to 90% load (the rest loaded untimed) and each tenth's time scaled from them; "
+                   f"{2 * qs} qf_count_key_value calls timed at 90% load (2^22 inserted, 2^22 disjoint keys) "
+                   f"and scaled to the {n_q} queries; reference host path restated in C, single thread"),
+        "inserts_mops": round(n / t_ins_est / 1e6, 3),
+        "queries_mops": round(n_q / t_q_est / 1e6, 3),
+        "seconds_timed": round(t_timed + t_p + t_d, 3),
+        "config1": cpu_config1(),
+        "cpu_model": cpu_model(),
         "nproc": os.cpu_count(),
     }
+
+
+def loadcurve_main(args):
+    """Rates at each load step, as the reference's test reports them
+    (src/test.cu:405-448, 789-873): an empty 2^28-slot table is filled in
+    steps of 5% of its slots up to 95% with splitmix64(seed) keys (each
+    step's keys inserted in batches of at most 2^24); after each step,
+    bulk_get of that step's keys (every one must be found, as the reference
+    aborts otherwise) and of as many keys never inserted (false lookups).
+    Mops/s of each per step, from hipEvents on the call stream."""
+    import numpy as np
+    import torch
+    from cqf_amd import gqf
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    nbits, npoints, top = args.nbits, 20, 19
+    step_n = (1 << nbits) // npoints
+    batch = 1 << args.batch_log2
+    keys = torch.empty(top * step_n, dtype=torch.int64, device="cuda")
+    other = torch.empty(top * step_n, dtype=torch.int64, device="cuda")
+    gqf.fill_splitmix64(keys, args.seed, 0, stream)
+    gqf.fill_splitmix64(other, args.seed, top * step_n, stream)
+    work = torch.empty(step_n, dtype=torch.int64, device="cuda")
+    ret = torch.empty(step_n, dtype=torch.int64, device="cuda")
+    f = gqf.GQF(nbits)
+    f.reserve(batch)
+    rows = []
+    for rep in range(args.warmup + 1):
+        f.reset(stream)
+        rows = []
+        for k in range(top):
+            seg = keys[k * step_n:(k + 1) * step_n]
+            work.copy_(seg)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            ev[0].record(stream)
+            for i in range(0, step_n, batch):
+                f.bulk_insert(work[i:i + batch], 0, stream)
+            ev[1].record(stream)
+            f.bulk_get(seg, ret, stream)
+            ev[2].record(stream)
+            found = bool((ret > 0).all().item())
+            ev2 = torch.cuda.Event(enable_timing=True)
+            ev2.record(stream)
+            f.bulk_get(other[k * step_n:(k + 1) * step_n], ret, stream)
+            ev[3].record(stream)
+            torch.cuda.synchronize()
+            fp = int((ret > 0).sum().item())
+            t_ins, t_ex, t_fa = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev2.elapsed_time(ev[3])
+            rows.append({"load_pct": 5 * (k + 1), "insert_mops": round(step_n / t_ins / 1e3, 1),
+                         "exist_lookup_mops": round(step_n / t_ex / 1e3, 1),
+                         "false_lookup_mops": round(step_n / t_fa / 1e3, 1),
+                         "insert_roofline_frac": round(step_n / t_ins * 1e3 * INSERT_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+                         "lookup_roofline_frac": round(step_n / t_ex * 1e3 * QUERY_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+                         "all_found": found, "false_positives": fp})
+    status = f.status()
+    f.close()
+    ins = [r["insert_mops"] for r in rows[:18]]
+    out = {"metric": "per-load-step Minserts/s and Mlookups/s (existing / false) at 2^28 slots, 5% steps to 95%",
+           "value": round(float(np.mean(ins)), 1), "unit": "Mops/s (mean insert rate over the steps to 90%)",
+           "n_gpus": 1, "steps": 1, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
+           "dtype": "u64", "data": "synthetic",
+           "config": {"workload": (f"2^{nbits} slots filled in 5% steps to 95% with splitmix64(seed={args.seed}) "
+                                   f"keys ({step_n} per step, batches of at most 2^{args.batch_log2}); after each "
+                                   f"step bulk_get of its keys and of {step_n} keys never inserted"),
+                      "nbits": nbits, "step_keys": step_n},
+           "curve": rows, "status": status}
+    print(json.dumps(out), flush=True)
 
 
 def zipf_main(args):
@@ -313,11 +443,15 @@ def groups_main(args):
 
 def serve_main(args):
     """Served stream: 64 filters of 2^22..2^30 slots admitted against an HBM
-    budget; Poisson arrivals of 60% get / 30% insert / 10% delete requests,
-    lognormal sizes (median 2^14 keys, sigma 1.0, clipped to [1, 2^22]) of
-    uniform splitmix64(6) keys.  The same stream is served by the batching
-    loop (several streams, coalescing) and by plain per-request bulk calls
-    on one stream; ops/s counts keys."""
+    budget that holds only part of them (evictions); Poisson arrivals of
+    60% get / 30% insert / 10% delete requests, lognormal sizes (median
+    2^14 keys, sigma 1.0, clipped to [1, 2^22]) of uniform splitmix64(6)
+    keys.  The native loop (qf_server_*) serves the stream at the overload
+    rate --serve-rate to measure its capacity, then the same requests at
+    50% and 80% of that capacity for steady-state latency; the plain path
+    (every request its own bulk call on one stream, all filters resident)
+    serves it at the overload rate.  ops/s counts keys; latency is arrival
+    to completion."""
     import numpy as np
     import torch
     from cqf_amd import serving as S
@@ -327,50 +461,64 @@ def serve_main(args):
     nbits, spec = W.serve_workload(args.serve_requests, nfilters=64, seed=6, q_lo=22, q_hi=30, median_log2=14,
                                    sigma=1.0, max_log2=22, rate_per_s=args.serve_rate)
     n_keys = sum(x[3] for x in spec)
+    max_batch = 1 << 22
+    total = sum(S.filter_bytes(nb, max_batch) for nb in nbits.values())
+    budget = int(total * args.serve_budget_frac)
 
-    def make():
-        return [S.Request(i, fid, op, W.splitmix64(6, n, skip=skip, device="cuda"), t)
+    def make(scale=1.0):
+        return [S.Request(i, fid, op, W.splitmix64(6, n, skip=skip, device="cuda"), t * scale)
                 for i, (t, fid, op, n, skip) in enumerate(spec)]
 
-    max_batch = 1 << 22
-    results = {}
-    for rep in range(args.warmup + 1):
-        reqs = make()
-        torch.cuda.synchronize()
-        srv = S.Server(budget_bytes=int(args.serve_budget_gb * 2 ** 30), n_streams=args.serve_streams,
-                       max_batch_keys=max_batch, max_inflight=args.serve_inflight)
+    def serve(scale):
+        reqs = make(scale)
+        srv = S.Server(budget_bytes=budget, n_streams=args.serve_streams, max_batch_keys=max_batch,
+                       max_inflight=args.serve_inflight)
         for fid, nb in nbits.items():
             srv.add_filter(fid, nb)
         wall = srv.run(reqs)
-        torch.cuda.synchronize()
         st = S.statuses(srv)
-        results["served"] = (wall, S.latency_stats(reqs), srv.batches, srv.evictions, srv.admissions,
-                             sum(1 for v in st.values() if v))
+        rec = {"wall_s": round(wall, 4), "mops": round(n_keys / wall / 1e6, 3),
+               "offered_req_s": round(len(spec) / (spec[-1][0] * scale), 1),
+               "served_req_s": round(len(spec) / wall, 1), "latency": S.latency_stats(reqs),
+               "batches": srv.batches, "evictions": srv.evictions, "admissions": srv.admissions,
+               "filters_with_status": sum(1 for v in st.values() if v)}
         srv.close()
-        del srv
-        reqs = make()
+        del reqs
         torch.cuda.synchronize()
-        wall_p = S.run_plain(nbits, reqs, max_batch)
-        results["plain"] = (wall_p, S.latency_stats(reqs))
-    wall, lat, batches, ev, adm, bad_status = results["served"]
-    wall_p, lat_p = results["plain"]
+        return rec
+
+    for _ in range(args.warmup):
+        serve(1.0)
+    over = serve(1.0)
+    cap_req_s = over["served_req_s"]
+    at_load = []
+    for frac in (0.5, 0.8):
+        # arrival times stretched so the offered rate is frac x capacity
+        scale = (len(spec) / spec[-1][0]) / (frac * cap_req_s)
+        rec = serve(scale)
+        rec["rate_frac_of_capacity"] = frac
+        at_load.append(rec)
+    reqs = make()
+    torch.cuda.synchronize()
+    wall_p = S.run_plain(nbits, reqs, max_batch)
+    plain = {"wall_s": round(wall_p, 4), "mops": round(n_keys / wall_p / 1e6, 3),
+             "latency": S.latency_stats(reqs)}
+    del reqs
     out = {
         "metric": "served ops/s (keys) and p50/p99 request latency, 64 resident filters, mixed get/insert/delete",
-        "value": round(n_keys / wall / 1e6, 3), "unit": "Mops/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u64", "data": "synthetic",
+        "value": over["mops"], "unit": "Mops/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": round(over["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": (f"{len(spec)} requests, Poisson {args.serve_rate:.0f}/s, 60/30/10 get/insert/delete, "
                                 "lognormal sizes median 2^14 sigma 1.0 clipped [1, 2^22], 64 filters q 22..30 "
                                 "(splitmix64 seed 6)"),
-                   "requests": len(spec), "keys": n_keys, "budget_gb": args.serve_budget_gb,
-                   "filters_slots": int(sum(1 << q for q in nbits.values()))},
-        "served": {"mops": round(n_keys / wall / 1e6, 3), "wall_s": round(wall, 4), "latency": lat,
-                   "batches": batches, "evictions": ev, "admissions": adm, "streams": args.serve_streams,
-                   "max_inflight": args.serve_inflight,
-                   "filters_with_status": bad_status},
-        "plain_per_request": {"mops": round(n_keys / wall_p / 1e6, 3), "wall_s": round(wall_p, 4),
-                              "latency": lat_p},
-        "speedup_vs_plain": round(wall_p / wall, 3),
+                   "requests": len(spec), "keys": n_keys, "budget_gb": round(budget / 2 ** 30, 2),
+                   "filters_gb": round(total / 2 ** 30, 2), "streams": args.serve_streams,
+                   "max_inflight": args.serve_inflight, "filters_slots": int(sum(1 << q for q in nbits.values()))},
+        "served_overload": over,
+        "served_at_load": at_load,
+        "plain_per_request": plain,
+        "speedup_vs_plain": round(wall_p / over["wall_s"], 3),
     }
     print(json.dumps(out), flush=True)
 
@@ -579,6 +727,9 @@ def main():
         return
     if args.workload == "table":
         table_main(args)
+        return
+    if args.workload == "loadcurve":
+        loadcurve_main(args)
         return
 
     import numpy as np

@@ -1136,6 +1136,25 @@ static double now_s(void)
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Seconds of n sequential qf_insert calls (count 1, keys hashed) on an
+ * existing table: the reference host path's insert loop. */
+double orc_time_inserts(orc_qf *qf, const uint64_t *keys, uint64_t n)
+{
+    double t0 = now_s();
+    for (uint64_t i = 0; i < n; i++) orc_qf_insert(qf, keys[i], 1, ORC_NO_LOCK);
+    return now_s() - t0;
+}
+
+/* Seconds of n qf_count_key_value calls; *hits = keys found. */
+double orc_time_queries(const orc_qf *qf, const uint64_t *keys, uint64_t n, uint64_t *hits)
+{
+    double t0 = now_s();
+    uint64_t h = 0;
+    for (uint64_t i = 0; i < n; i++) h += orc_qf_count_key_value(qf, keys[i], ORC_NO_LOCK) > 0;
+    *hits = h;
+    return now_s() - t0;
+}
+
 double orc_time_insert_query(int nbits, const uint64_t *keys, uint64_t n_ins,
                              const uint64_t *qkeys, uint64_t n_q, uint64_t *hits,
                              double *t_insert, double *t_query)

@@ -132,7 +132,9 @@ uint64_t orc_cluster_census(orc_qf *qf, uint64_t *lengths, uint64_t cap);
 /* ---- key generators ---- */
 void     orc_splitmix64_fill(uint64_t seed, uint64_t skip, uint64_t n, uint64_t *out);
 
-/* ---- timing helper for the CPU baseline ---- */
+/* ---- timing helpers for the CPU baseline ---- */
+double   orc_time_inserts(orc_qf *qf, const uint64_t *keys, uint64_t n);
+double   orc_time_queries(const orc_qf *qf, const uint64_t *keys, uint64_t n, uint64_t *hits);
 double   orc_time_insert_query(int nbits, const uint64_t *keys, uint64_t n_ins,
                                const uint64_t *qkeys, uint64_t n_q, uint64_t *hits,
                                double *t_insert, double *t_query);

@@ -82,6 +82,10 @@ def lib():
         L.orc_bulk_get_misses.argtypes = [P, _u64p, ctypes.c_uint64]
         L.orc_bulk_get_misses.restype = ctypes.c_uint64
         L.orc_build_canonical.argtypes = [P, _u64p, _u64p, ctypes.c_uint64]
+        L.orc_time_inserts.argtypes = [P, _u64p, ctypes.c_uint64]
+        L.orc_time_inserts.restype = ctypes.c_double
+        L.orc_time_queries.argtypes = [P, _u64p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_time_queries.restype = ctypes.c_double
         L.orc_build_canonical.restype = ctypes.c_int
         L.orc_enumerate.argtypes = [P, _u64p, _u64p, ctypes.c_uint64]
         L.orc_enumerate.restype = ctypes.c_uint64
@@ -242,6 +246,18 @@ class OracleQF:
     def bulk_get_misses(self, vals):
         vals = np.ascontiguousarray(vals, dtype=np.uint64)
         return int(lib().orc_bulk_get_misses(ctypes.byref(self._s), _p(vals), vals.size))
+
+    def time_inserts(self, keys):
+        """Seconds of one sequential qf_insert per key (keys hashed)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        return float(lib().orc_time_inserts(ctypes.byref(self._s), _p(keys), keys.size))
+
+    def time_queries(self, keys):
+        """(seconds, keys found) of one qf_count_key_value per key."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        hits = ctypes.c_uint64()
+        t = float(lib().orc_time_queries(ctypes.byref(self._s), _p(keys), keys.size, ctypes.byref(hits)))
+        return t, int(hits.value)
 
     def build_canonical(self, hashes, counts):
         hashes = np.ascontiguousarray(hashes, dtype=np.uint64)
