@@ -200,6 +200,9 @@ bool reduce_by_counting(Filter* f, uint64_t n, uint64_t* keys, int bits, hipStre
   return true;
 }
 
+void run_mutation(Filter* f, Workspace& w, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool reduce,
+                  hipStream_t s);
+
 void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool reduce, hipStream_t s) {
   if (n == 0) return;  // the reference's grid size underflows here
   Filter* f = lookup(qf);
@@ -220,7 +223,14 @@ void mutate(QF* qf, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool re
     return;
   }
   ensure_workspace(f, n);
-  Workspace& w = f->ws;
+  run_mutation(f, f->ws, n, keys, flags, mode, reduce, s);
+}
+
+// The bulk pipeline of one batch (n <= GQF_MAX_PIECE) on a workspace sized
+// for it; allocates nothing itself (the counting reduce allocates its table
+// on first use, so reduce batches take workspaces that may allocate).
+void run_mutation(Filter* f, Workspace& w, uint64_t n, uint64_t* keys, uint8_t flags, int mode, bool reduce,
+                  hipStream_t s) {
   const bool hash_it = !(flags & QF_KEY_IS_HASH);
   const int vb = (int)f->host.value_bits;
   const int bits = hash_it ? (int)(f->host.key_bits + f->host.value_bits) : 64;
@@ -539,6 +549,114 @@ extern "C" void bulk_get_async(QF* qf, uint64_t nvals, uint64_t* vals, uint64_t*
 }
 
 // ---------------------------------------------------------------------------
+// bulk_insert without allocation
+//
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops, harnesses that forbid hipMalloc): the workspace of
+// the pipeline -- sort buffer and histograms, region table and lists, wide
+// flags -- is carved out of one file-scope device array instead of the
+// filter's own workspace.  Each GPU holds one copy of the array for the
+// life of the process; every call on that GPU shares it, so calls that
+// overlap on one GPU race (issue them on one stream, or synchronise).
+namespace {
+
+constexpr uint64_t kNoAllocMaxKeys = 1ULL << 20;       // keys per call
+constexpr uint64_t kNoAllocMaxRegions = 1200000;       // rebuild regions of the filter (2^30 slots at 16 blocks)
+constexpr uint64_t kNoAllocAlign = 256;
+
+constexpr uint64_t na_round(uint64_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+// sub-buffers, in order
+constexpr uint64_t kNaTmp = na_round(kNoAllocMaxKeys * 8);
+constexpr uint64_t kNaHist = na_round(1024ULL * (kNoAllocMaxKeys / 4096 + 64) * 4);  // gqf_sort_hist_words(max) words
+constexpr uint64_t kNaTotals = na_round(1024 * 4);
+constexpr uint64_t kNaSlow = na_round(GQF_SORT_MAX_BUCKETS * 4);
+constexpr uint64_t kNaStart = na_round((kNoAllocMaxRegions + 2) * 4);
+constexpr uint64_t kNaList = na_round(2 * ((kNoAllocMaxRegions + 1) / 2 + 1) * 4);
+constexpr uint64_t kNaCount = na_round(2 * 4);
+constexpr uint64_t kNaWideFlags = na_round(kNoAllocMaxRegions + 1);
+constexpr uint64_t kNaWideCount = na_round(GQF_WIDE_WORDS * 4);
+constexpr uint64_t kNoAllocBytes =
+    kNaTmp + kNaHist + kNaTotals + kNaSlow + kNaStart + kNaList + kNaCount + kNaWideFlags + kNaWideCount;
+
+// The scratch array.  In host code `scratch` is only a stand-in for the
+// symbol: its device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once per
+// device, with that device current); nullptr on failure
+void* scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      note_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+__global__ void gqf_zero_bytes(uint8_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" int bulk_insert_noalloc(QF* qf, uint64_t nvals, uint64_t* keys, uint8_t flags, void* stream) {
+  if (nvals == 0) return 0;
+  Filter* f = lookup(qf);
+  if (!f) return -1;
+  if (nvals > kNoAllocMaxKeys) {
+    fprintf(stderr, "libgqf: bulk_insert_noalloc takes at most %llu keys per call (got %llu)\n",
+            (unsigned long long)kNoAllocMaxKeys, (unsigned long long)nvals);
+    return -2;
+  }
+  if (f->host.nregions > kNoAllocMaxRegions) {
+    fprintf(stderr, "libgqf: bulk_insert_noalloc takes filters of at most %llu rebuild regions (this one has %llu)\n",
+            (unsigned long long)kNoAllocMaxRegions, (unsigned long long)f->host.nregions);
+    return -2;
+  }
+  if (gqf_sort_hist_words(nvals) * 4 > kNaHist) return -2;  // cannot happen for nvals <= kNoAllocMaxKeys
+  uint8_t* base = (uint8_t*)scratch_address();
+  if (!base) return -3;
+  Workspace w;  // a view into scratch: nothing here is freed
+  uint8_t* p = base;
+  w.cap = kNoAllocMaxKeys;
+  w.tmp = (uint64_t*)p;                 p += kNaTmp;
+  w.hist = (uint32_t*)p;                p += kNaHist;
+  w.totals = (uint32_t*)p;              p += kNaTotals;
+  w.slow_buckets = (uint32_t*)p;        p += kNaSlow;
+  w.region_start = (uint32_t*)p;        p += kNaStart;
+  w.region_list = (uint32_t*)p;         p += kNaList;
+  w.list_count = (uint32_t*)p;          p += kNaCount;
+  w.wide_flags = (uint8_t*)p;           p += kNaWideFlags;
+  w.wide_count = (uint32_t*)p;
+  const hipStream_t s = (hipStream_t)stream;
+  // the flags the kernels expect at zero: the last call on this GPU left
+  // them so, but clear them anyway (a fill kernel, not a memset node, so
+  // the call can be captured into a graph)
+  hipLaunchKernelGGL(gqf_zero_bytes, dim3(64), dim3(256), 0, s, (uint8_t*)w.slow_buckets, (uint64_t)kNaSlow);
+  hipLaunchKernelGGL(gqf_zero_bytes, dim3(64), dim3(256), 0, s, w.wide_flags, f->host.nregions + 1);
+  hipLaunchKernelGGL(gqf_zero_bytes, dim3(1), dim3(256), 0, s, (uint8_t*)w.wide_count,
+                     (uint64_t)GQF_WIDE_WORDS * 4);
+  gqf_prof_suppress(true);  // per-kernel timing would create events
+  run_mutation(f, w, nvals, keys, flags, kModeInsert, false, s);
+  gqf_prof_suppress(false);
+  const hipError_t e = hipGetLastError();
+  note_error(e, "bulk_insert_noalloc");
+  return e == hipSuccess ? 0 : -3;
+}
+
+// ---------------------------------------------------------------------------
 // sequenced batches
 
 namespace {
@@ -811,7 +929,12 @@ static bool capturing(hipStream_t s) {
   return s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
 
+// set around calls that must not create events or grow the record vectors
+static thread_local bool g_prof_suppressed = false;
+void gqf_prof_suppress(bool on) { g_prof_suppressed = on; }
+
 void gqf_prof_begin(const char* name, hipStream_t s) {
+  if (g_prof_suppressed) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (!g_prof_on || capturing(s)) return;
   ProfRec r;
@@ -823,6 +946,7 @@ void gqf_prof_begin(const char* name, hipStream_t s) {
 }
 
 void gqf_prof_end(const char* name, hipStream_t s) {
+  if (g_prof_suppressed) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (!g_prof_on || capturing(s)) return;
   auto it = g_prof_open.find(name);

@@ -182,6 +182,8 @@ void gqf_launch_census(const QF* qf_dev, uint64_t nblocks, uint64_t* used, uint3
 // the completed intervals of one kernel name.
 void gqf_prof_begin(const char* name, hipStream_t s);
 void gqf_prof_end(const char* name, hipStream_t s);
+// while on (this thread), gqf_prof_begin / _end do nothing
+void gqf_prof_suppress(bool on);
 
 // ---------------------------------------------------------------------------
 // Filter groups (gqf_group.hip): several filters served by shared launches.

@@ -40,7 +40,7 @@ EXPORTS = [
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
     "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
     "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu", "gqf_group_set_wait_bound",
-    "qf_batch_pieces", "qf_check_layout",
+    "qf_batch_pieces", "bulk_insert_noalloc", "qf_check_layout",
     "qf_server_create", "qf_server_add_filter", "qf_server_submit", "qf_server_run", "qf_server_wait",
     "qf_server_request_times", "qf_server_get_stats", "qf_server_filter", "qf_server_destroy",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
@@ -136,6 +136,8 @@ def load_library(path=LIB_PATH):
     L.gqf_group_set_wait_bound.restype = ctypes.c_int
     L.qf_check_layout.argtypes = [_vp, ctypes.POINTER(_u64)]
     L.qf_check_layout.restype = ctypes.c_int
+    L.bulk_insert_noalloc.argtypes = [_vp, _u64, _vp, ctypes.c_uint8, _vp]
+    L.bulk_insert_noalloc.restype = ctypes.c_int
     L.qf_batch_pieces.argtypes = [_u64, _u64, ctypes.POINTER(_u64), ctypes.c_uint32]
     L.qf_batch_pieces.restype = ctypes.c_uint32
     L.qf_malloc_device_kv.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
@@ -238,6 +240,12 @@ class GQF:
     # -- bulk API (tensors are int64 device tensors)
     def bulk_insert(self, keys, flags=0, stream=None):
         self._lib.bulk_insert_async(self.handle, keys.numel(), _ptr(keys), flags, _stream_ptr(stream))
+
+    def bulk_insert_noalloc(self, keys, flags=0, stream=None):
+        """bulk_insert through the entry point that allocates nothing;
+        returns its status (0, or < 0 with nothing launched)."""
+        return int(self._lib.bulk_insert_noalloc(self.handle, keys.numel(), _ptr(keys), flags,
+                                                 _stream_ptr(stream)))
 
     def bulk_insert_reduce(self, keys, flags=0, stream=None):
         self._lib.bulk_insert_reduce_async(self.handle, keys.numel(), _ptr(keys), flags, _stream_ptr(stream))

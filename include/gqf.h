@@ -131,6 +131,19 @@ void bulk_insert_reduce_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t fl
 void bulk_delete_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_get_async(QF *qf, uint64_t nvals, uint64_t *vals, uint64_t *returns, void *stream);
 
+/* bulk_insert_async without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same result and key contract as bulk_insert.  Its workspace is a fixed
+ * device array of the library (20.5 MB per GPU, held until the process
+ * ends), shared by every call on that GPU: calls that overlap on one GPU
+ * race, so issue them on one stream or synchronise between them.  Takes at
+ * most 2^20 keys per call and filters of at most 1,200,000 rebuild regions
+ * (2^30 slots at the smallest region size); returns 0, -1 for an unknown
+ * handle, -2 for a batch or filter beyond those limits (nothing launched),
+ * -3 for a HIP error. */
+int bulk_insert_noalloc(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
+
 /* ---- sequenced batches (no reference counterpart) -------------------
  * One filter's requests, in arrival order, as a few launches instead of
  * one bulk call each, with the results of the calls one by one.  For
