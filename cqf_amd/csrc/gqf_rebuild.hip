@@ -31,6 +31,9 @@
 // window, in ascending region order, after the phase.
 #include <stdlib.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "gqf_internal.h"
 #include "gqf_launch.h"
 
@@ -48,7 +51,7 @@ enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 template <int W>
 struct Smem {
   uint32_t sin[W * 17];   // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
-  uint32_t sout[W * 17];  // new slots, same rows
+  uint32_t sout[(W + 1) * 17];  // new slots, same rows, and a spare row (a shift past row W - 1 lands there)
   uint64_t run_in[W];
   // arrays whose lifetimes do not overlap share space
   union {
@@ -154,6 +157,52 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
                                                    int32_t st_o, uint32_t old_len, int32_t st_n, uint32_t h_first) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
+  // Inserts (kAdd): one or two items, with distinct hashes, on a run of at
+  // most 8 count-1 counters (strictly ascending bytes, see below) -- nearly
+  // every quotient of a batch.  The new run is the old bytes and the new
+  // ones merged, an old byte before an equal new one (r r is the count-2
+  // counter of r).  Straight-line code, the same on every lane: bytes past
+  // the run read as sentinels 0x100 + k (ascending, above every byte), and
+  // writes of absent bytes go to the lane's byte of the spare row.
+  if (src.kind == kAdd && it1 - it0 <= 2 && old_len <= 8) {
+    const uint32_t n = it1 - it0;
+    const uint32_t r0 = h_first & 0xFFu;
+    const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : 0x200u;
+    const uint32_t bo = sin_off(st_o);
+    const uint32_t ro = 64u - (uint32_t)(st_o & 63);  // bytes of the run's first row
+    uint32_t ob[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t b = in[bo + (uint32_t)k + ((uint32_t)k >= ro ? 4u : 0u)];
+      ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
+    }
+    bool simple = n == 1 || r1 > r0;
+#pragma unroll
+    for (int k = 1; k < 8; k++) simple = simple && ob[k] > ob[k - 1];
+    if (simple) {
+      if (WRITE) {
+        const uint32_t bn = sin_off(st_n);
+        const uint32_t rn = 64u - (uint32_t)(st_n & 63);
+        const uint32_t spare = (uint32_t)(W * 68) + (uint32_t)lane_id();
+        auto put_at = [&](uint32_t at, bool live, uint32_t v) {
+          const uint32_t a = bn + at + (at >= rn ? 4u : 0u);
+          out[live ? a : spare] = (uint8_t)v;
+        };
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          put_at((uint32_t)k + (r0 < ob[k] ? 1u : 0u) + (r1 < ob[k] ? 1u : 0u), (uint32_t)k < old_len, ob[k]);
+        uint32_t a0 = 0, a1 = 1;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          a0 += ob[k] <= r0 ? 1u : 0u;
+          a1 += ob[k] <= r1 ? 1u : 0u;
+        }
+        put_at(a0, true, r0);
+        put_at(a1, n == 2, r1);
+      }
+      return old_len + n;
+    }
+  }
   // Common case: one item adding or removing 1 on a run (at most 8 slots) of
   // count-1 counters.  Such a run's bytes strictly ascend -- a counter of
   // count >= 2 repeats or drops a byte (r r, r 0 r, 0 0 0, r [0] d.. r,
@@ -516,6 +565,28 @@ __device__ inline uint64_t range_mask(int lo, int hi) {  // bits [lo, hi) of a w
   return gqf_mask((uint64_t)hi) & ~gqf_mask((uint64_t)lo);
 }
 
+// Bit b of x as 0 or ~0u, and min(x, 1): each one VALU instruction.  As
+// asm the optimizer keeps them as values (it would otherwise rewrite the
+// masks applied with them into compare-and-select chains).
+// f(integral_constant<int, i>) for i = 0, 1, ..., in order, as straight-line code
+template <typename F, int... I>
+__device__ __forceinline__ void for_each_slot(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int b>
+__device__ __forceinline__ uint32_t bit_mask(uint32_t x) {
+  static_assert(b >= 0 && b < 32, "bit index");
+  uint32_t r;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(r) : "v"(x), "i"(b));
+  return r;
+}
+__device__ __forceinline__ uint32_t min1(uint32_t x) {
+  uint32_t r;
+  asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // ORs the low `len` bits of `bits` into the window bit array at position d
 // (len <= 64; the bits may straddle two words)
 __device__ inline void or_bits(uint64_t* arr, int32_t d, uint64_t bits, int len) {
@@ -659,14 +730,16 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
     }
     // The events, in order: each one's new start; the slots of the old runs
     // they replace (`skip`, with what an earlier block's last event covers);
-    // and the shift each event position adds, as two bit planes (a total
-    // of 1..3 per position; more sends the block to the general loop below,
-    // as does a shift that could reach past the next block).
+    // and the shift each event position adds, in 2-bit fields (ev[0]:
+    // positions 0..31, ev[1]: 32..63).  A position whose fields would pass
+    // 3, or a shift above kShiftFast, sends the block to the general loop
+    // below.
+    constexpr int kShiftFast = 56;
     uint64_t skip = cur > base ? gqf_mask((uint64_t)min(64, cur - base)) : 0ULL;
-    uint64_t ev0 = 0, ev1 = 0;
+    uint64_t ev[2] = {0ULL, 0ULL};
     bool general = false;
     {
-      int32_t delta = delta_in, grow = 0;
+      int32_t delta = delta_in, emax = delta_in;
       int pos = 0;
       for (uint32_t t = e0; t < e1; t++) {
         const int32_t st = sm.tst[t];
@@ -676,68 +749,82 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
         sm.tsn[t] = (int16_t)(st + delta);
         const int32_t d = (int32_t)sm.tnl[t] - (int32_t)sm.tol[t];
         delta += d;
-        grow += d;
+        if (delta > emax) emax = delta;
         pos = rel;
         if (d < 0) general = true;  // inserts never shorten a run; kept general all the same
         if (d > 0) {
-          const int now = (int)((ev0 >> rel) & 1ULL) + 2 * (int)((ev1 >> rel) & 1ULL) + d;
-          if (now > 3) general = true;
-          const uint64_t bit = 1ULL << rel;
-          ev0 = (ev0 & ~bit) | ((now & 1) ? bit : 0ULL);
-          ev1 = (ev1 & ~bit) | ((now & 2) ? bit : 0ULL);
+          const int sh = 2 * (rel & 31);
+          const uint64_t h = rel < 32 ? ev[0] : ev[1];
+          if ((int)((h >> sh) & 3ULL) + d > 3) general = true;
+          const uint64_t add = (uint64_t)(d & 3) << sh;
+          if (rel < 32) ev[0] += add; else ev[1] += add;
         }
         const int tol = (int)sm.tol[t];
         if (tol) skip |= range_mask(rel, min(64, rel + tol));
         const int32_t pe = st + tol;
         if (pe > cur) cur = pe;
       }
-      if (delta_in + grow >= 64) general = true;
+      if (emax > kShiftFast) general = true;
     }
     if (!general) {
-      // One straight pass over the block's 64 slots (the same code on every
-      // lane): slot i, if it holds an untouched byte, goes to i + shift;
-      // the shift grows at event positions and drops by one (not below 0)
-      // at each valid empty slot.  The run-end and used bits of the
-      // destinations collect in registers for this block and the next.
+      // One straight pass over the block's 64 slots, the same code on every
+      // lane.  e, the shift at slot i, grows by the slot's event fields; an
+      // untouched byte at i goes to i + e; after the slot, e drops by one
+      // (not below 0) if the slot is a valid empty.  Slot i of the block is
+      // used afterwards iff it was used or e > 0 there (the shifted items
+      // fill it), so the used mask needs no scatter.  The run ends of the
+      // moved bytes collect in eight 64-bit windows: slot i's lands in
+      // window i / 8 at bit e + i % 8 (<= kShiftFast + 7 < 64).
       uint32_t w[16];
       const uint32_t* row = sm.sin + 17 * j;
 #pragma unroll
       for (int k = 0; k < 16; k++) w[k] = row[k];
       const uint64_t copy = used & ~skip;
-      int32_t d = delta_in;
-      uint64_t rlo = 0, rhi = 0, ulo = 0, uhi = 0;
-      uint8_t* dst = out + sin_off(base);  // row j; row j + 1 starts 68 bytes on
-      // a byte moved past this block lands in block j + 1, which b* always
-      // covers (such a shift leaves the block > 0); checked so no write
-      // leaves the rows
-      const bool next_ok = j + 1 < bstar;
+      const uint64_t runc = run_j & copy;
+      uint32_t e = (uint32_t)delta_in;
+      uint32_t nz[2] = {0u, 0u};
+      uint64_t rw[8];
 #pragma unroll
-      for (int i = 0; i < 64; i++) {
-        d += (int32_t)((ev0 >> i) & 1ULL) + 2 * (int32_t)((ev1 >> i) & 1ULL);
-        if ((copy >> i) & 1ULL) {
-          const int p = i + d;  // < 128
-          const uint8_t byte = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
-          const uint64_t b = 1ULL << (p & 63);
-          const uint64_t r = ((run_j >> i) & 1ULL) ? b : 0ULL;
-          if (p < 64) {
-            dst[p] = byte;
-            ulo |= b;
-            rlo |= r;
-          } else if (next_ok) {
-            dst[p + 4] = byte;
-            uhi |= b;
-            rhi |= r;
-          } else {
-            uhi |= b;  // flagged below
-          }
-        }
-        if ((zero_ok >> i) & 1ULL) d = d > 0 ? d - 1 : 0;
+      for (int k = 0; k < 8; k++) rw[k] = 0ULL;
+      const uint32_t cw[2] = {(uint32_t)copy, (uint32_t)(copy >> 32)};
+      const uint32_t rc[2] = {(uint32_t)runc, (uint32_t)(runc >> 32)};
+      const uint32_t zw[2] = {(uint32_t)zero_ok, (uint32_t)(zero_ok >> 32)};
+      const uint32_t ew[4] = {(uint32_t)ev[0], (uint32_t)(ev[0] >> 32), (uint32_t)ev[1], (uint32_t)(ev[1] >> 32)};
+      uint8_t* dst = out + sin_off(base);  // row j; row j + 1 starts 68 bytes on
+      auto slot = [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        e += (ew[i >> 4] >> (2 * (i & 15))) & 3u;
+        nz[i >> 5] |= min1(e) << (i & 31);
+        const uint32_t sft = e + (uint32_t)(i & 7);      // < 64
+        const uint32_t pdst = sft + (uint32_t)(i & ~7);  // i + e < 128
+        // row j + 1 starts 68 bytes on; no shift reaches it from i < 8
+        const uint32_t off = i + kShiftFast < 64 ? pdst : pdst + (((pdst >> 6) & 1u) << 2);
+        // slots that move nothing write to the row's pad byte (offset 64),
+        // which no row read takes
+        const uint32_t cm = bit_mask<i & 31>(cw[i >> 5]);
+        const uint32_t woff = (off & cm) | (64u & ~cm);
+        dst[woff] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        const uint32_t rm = bit_mask<i & 31>(rc[i >> 5]);
+        const uint64_t one = 1ULL << sft;
+        rw[i >> 3] |= one & ((uint64_t)rm << 32 | rm);
+        const uint32_t z = (zw[i >> 5] >> (i & 31)) & 1u;
+        e = e > z ? e - z : 0u;
+      };
+      for_each_slot(slot, std::make_integer_sequence<int, 64>{});
+      // e is now the shift leaving the block: the bytes moved into block
+      // j + 1 fill its first e slots
+      uint64_t rlo = 0, rhi = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        rlo |= rw[k] << (8 * k);
+        if (k) rhi |= rw[k] >> (64 - 8 * k);
       }
+      const uint64_t ulo = (used | ((uint64_t)nz[1] << 32 | nz[0])) & (used | gqf_mask((uint64_t)vlim));
       atomicOr((unsigned long long*)&sm.used_out[j], ulo);
       atomicOr((unsigned long long*)&sm.run_out[j], rlo);
-      if (uhi) {
-        if (next_ok) {
-          atomicOr((unsigned long long*)&sm.used_out[j + 1], uhi);
+      if (e) {
+        if (j + 1 < bstar) {  // always, by the choice of b*
+          atomicOr((unsigned long long*)&sm.used_out[j + 1], gqf_mask((uint64_t)e));
           atomicOr((unsigned long long*)&sm.run_out[j + 1], rhi);
         } else {
           atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
