@@ -42,7 +42,7 @@ namespace {
 constexpr int kWinFast = 64;          // region blocks + push blocks
 constexpr int kWinMid = 128;          // first fallback window (80 push blocks)
 constexpr int kWinWide = 256;         // last fallback window (208 push blocks)
-constexpr int kItemCap = 232;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
+constexpr int kItemCap = 256;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
 enum RebuildMode : int { kInsert = 0, kDelete = 1, kSeq = 2 };
