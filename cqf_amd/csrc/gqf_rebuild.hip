@@ -565,15 +565,16 @@ __device__ inline uint64_t range_mask(int lo, int hi) {  // bits [lo, hi) of a w
   return gqf_mask((uint64_t)hi) & ~gqf_mask((uint64_t)lo);
 }
 
-// Bit b of x as 0 or ~0u, and min(x, 1): each one VALU instruction.  As
-// asm the optimizer keeps them as values (it would otherwise rewrite the
-// masks applied with them into compare-and-select chains).
 // f(integral_constant<int, i>) for i = 0, 1, ..., in order, as straight-line code
 template <typename F, int... I>
 __device__ __forceinline__ void for_each_slot(F& f, std::integer_sequence<int, I...>) {
   (f(std::integral_constant<int, I>{}), ...);
 }
 
+// Bit b of x as 0 or ~0u, bit b as 0 or 1, min(x, 1) and (x << sh) | acc:
+// each one VALU instruction.  As asm the optimizer keeps them as values (it
+// would otherwise rewrite the masks and bits applied with them into
+// compare-and-select chains).
 template <int b>
 __device__ __forceinline__ uint32_t bit_mask(uint32_t x) {
   static_assert(b >= 0 && b < 32, "bit index");
@@ -581,9 +582,27 @@ __device__ __forceinline__ uint32_t bit_mask(uint32_t x) {
   asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(r) : "v"(x), "i"(b));
   return r;
 }
+template <int b>
+__device__ __forceinline__ uint32_t bit_u(uint32_t x) {  // bit b of x, 0 or 1
+  static_assert(b >= 0 && b < 32, "bit index");
+  uint32_t r;
+  asm("v_bfe_u32 %0, %1, %2, 1" : "=v"(r) : "v"(x), "i"(b));
+  return r;
+}
 __device__ __forceinline__ uint32_t min1(uint32_t x) {
   uint32_t r;
   asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ uint32_t lshl_or(uint32_t x, uint32_t sh, uint32_t acc) {  // (x << sh) | acc
+  uint32_t r;
+  asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(sh), "v"(acc));
+  return r;
+}
+template <int sh>
+__device__ __forceinline__ uint32_t lshl_or_c(uint32_t x, uint32_t acc) {  // (x << sh) | acc
+  uint32_t r;
+  asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "i"(sh), "v"(acc));
   return r;
 }
 
@@ -733,8 +752,9 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
     // and the shift each event position adds, in 2-bit fields (ev[0]:
     // positions 0..31, ev[1]: 32..63).  A position whose fields would pass
     // 3, or a shift above kShiftFast, sends the block to the general loop
-    // below.
-    constexpr int kShiftFast = 56;
+    // below (at 2^24 inserts into 2^28 slots, one block in 60,000 passes
+    // 24).
+    constexpr int kShiftFast = 24;
     uint64_t skip = cur > base ? gqf_mask((uint64_t)min(64, cur - base)) : 0ULL;
     uint64_t ev[2] = {0ULL, 0ULL};
     bool general = false;
@@ -773,8 +793,8 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       // (not below 0) if the slot is a valid empty.  Slot i of the block is
       // used afterwards iff it was used or e > 0 there (the shifted items
       // fill it), so the used mask needs no scatter.  The run ends of the
-      // moved bytes collect in eight 64-bit windows: slot i's lands in
-      // window i / 8 at bit e + i % 8 (<= kShiftFast + 7 < 64).
+      // moved bytes collect in eight 32-bit windows: slot i's lands in
+      // window i / 8 at bit e + i % 8 (<= kShiftFast + 7 < 32).
       uint32_t w[16];
       const uint32_t* row = sm.sin + 17 * j;
 #pragma unroll
@@ -783,9 +803,9 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       const uint64_t runc = run_j & copy;
       uint32_t e = (uint32_t)delta_in;
       uint32_t nz[2] = {0u, 0u};
-      uint64_t rw[8];
+      uint32_t rw[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) rw[k] = 0ULL;
+      for (int k = 0; k < 8; k++) rw[k] = 0u;
       const uint32_t cw[2] = {(uint32_t)copy, (uint32_t)(copy >> 32)};
       const uint32_t rc[2] = {(uint32_t)runc, (uint32_t)(runc >> 32)};
       const uint32_t zw[2] = {(uint32_t)zero_ok, (uint32_t)(zero_ok >> 32)};
@@ -794,19 +814,17 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       auto slot = [&](auto ic) {
         constexpr int i = decltype(ic)::value;
         e += (ew[i >> 4] >> (2 * (i & 15))) & 3u;
-        nz[i >> 5] |= min1(e) << (i & 31);
-        const uint32_t sft = e + (uint32_t)(i & 7);      // < 64
-        const uint32_t pdst = sft + (uint32_t)(i & ~7);  // i + e < 128
-        // row j + 1 starts 68 bytes on; no shift reaches it from i < 8
-        const uint32_t off = i + kShiftFast < 64 ? pdst : pdst + (((pdst >> 6) & 1u) << 2);
+        nz[i >> 5] = lshl_or_c<i & 31>(min1(e), nz[i >> 5]);
+        const uint32_t sft = e + (uint32_t)(i & 7);      // < 32
+        const uint32_t pdst = sft + (uint32_t)(i & ~7);  // i + e < 88
+        // row j + 1 starts 68 bytes on; no shift reaches it from i < 40
+        const uint32_t off = i + kShiftFast < 64 ? pdst : pdst + ((pdst >> 6) << 2);
         // slots that move nothing write to the row's pad byte (offset 64),
         // which no row read takes
         const uint32_t cm = bit_mask<i & 31>(cw[i >> 5]);
         const uint32_t woff = (off & cm) | (64u & ~cm);
         dst[woff] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
-        const uint32_t rm = bit_mask<i & 31>(rc[i >> 5]);
-        const uint64_t one = 1ULL << sft;
-        rw[i >> 3] |= one & ((uint64_t)rm << 32 | rm);
+        rw[i >> 3] = lshl_or(bit_u<i & 31>(rc[i >> 5]), sft, rw[i >> 3]);
         const uint32_t z = (zw[i >> 5] >> (i & 31)) & 1u;
         e = e > z ? e - z : 0u;
       };
@@ -816,15 +834,15 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       uint64_t rlo = 0, rhi = 0;
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        rlo |= rw[k] << (8 * k);
-        if (k) rhi |= rw[k] >> (64 - 8 * k);
+        rlo |= (uint64_t)rw[k] << (8 * k);
+        if (8 * k + 31 >= 64) rhi |= (uint64_t)rw[k] >> (64 - 8 * k);
       }
-      const uint64_t ulo = (used | ((uint64_t)nz[1] << 32 | nz[0])) & (used | gqf_mask((uint64_t)vlim));
-      atomicOr((unsigned long long*)&sm.used_out[j], ulo);
+      // the block's whole used mask (the new runs' slots and what block j - 1
+      // pushes in included): lane j is its only writer
+      sm.used_out[j] = (used | ((uint64_t)nz[1] << 32 | nz[0])) & (used | gqf_mask((uint64_t)vlim));
       atomicOr((unsigned long long*)&sm.run_out[j], rlo);
       if (e) {
         if (j + 1 < bstar) {  // always, by the choice of b*
-          atomicOr((unsigned long long*)&sm.used_out[j + 1], gqf_mask((uint64_t)e));
           atomicOr((unsigned long long*)&sm.run_out[j + 1], rhi);
         } else {
           atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
@@ -853,7 +871,6 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
               for (int k = 0; k < len; k++) out[sin_off(dd + k)] = in[sin_off(a + k)];
               const uint64_t lm = gqf_mask((uint64_t)len);
               or_bits(sm.run_out, dd, (run_j >> ra) & lm, len);
-              or_bits(sm.used_out, dd, lm, len);
             } else {
               atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
             }
@@ -876,6 +893,17 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
         if (pe > gcur) gcur = pe;
       }
       stretches(gcur, base + 64);
+      // used afterwards, as in the straight pass: slot i is used iff it was,
+      // or the shift reaching it is above 0
+      uint64_t mz = 0;
+      int32_t ee = delta_in;
+      uint32_t t = e0;
+      for (int i = 0; i < 64; i++) {
+        for (; t < e1 && sm.tst[t] - base == i; t++) ee += (int32_t)sm.tnl[t] - (int32_t)sm.tol[t];
+        if (ee > 0) mz |= 1ULL << i;
+        if ((zero_ok >> i) & 1ULL) ee = ee > 0 ? ee - 1 : 0;
+      }
+      sm.used_out[j] = (used | mz) & (used | gqf_mask((uint64_t)vlim));
     }
   }
   __syncthreads();
@@ -892,12 +920,6 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
     const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
     merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sn, src.hash(sm.items, sm.tit0[t]));
     atomicOr((unsigned long long*)&sm.run_out[en >> 6], 1ULL << (en & 63));
-    // the run's slots are used (at most 2 words for runs below 64 slots)
-    for (int32_t p = sn; p <= en;) {
-      const int len = min(64 - (p & 63), en - p + 1);
-      atomicOr((unsigned long long*)&sm.used_out[p >> 6], gqf_mask((uint64_t)len) << (p & 63));
-      p += len;
-    }
   }
   __syncthreads();
 
