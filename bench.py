@@ -370,8 +370,13 @@ def groups_main(args):
     filters = [gqf.GQF(q) for q, *_ in params]
     group = gqf.GQFGroup(filters)
     t_tune = 0.0
+    tuned_from = "measured"
     if not args.no_autotune:
         t_tune = time.perf_counter()
+        from cqf_amd import tuning as tuning_table
+        if os.path.exists(tuning_table.TABLE):
+            tuning_table.load()  # region size and digit from the committed table; workgroups per CU still timed
+            tuned_from = os.path.basename(tuning_table.TABLE)
         group.autotune(offsets)
         t_tune = time.perf_counter() - t_tune
     n = keys.numel()
@@ -429,7 +434,7 @@ def groups_main(args):
                    "genome_len": "max(1000, N_f // 4)"},
         "insert_call_ms": round(float(np.mean(ins)), 3), "query_call_ms": round(float(np.mean(qry)), 3),
         "queries_mops": round(n * args.steps / q_s / 1e6, 3),
-        "autotune_s": round(t_tune, 2), "tuning_region_blocks/digit": tuning,
+        "autotune_s": round(t_tune, 2), "tuning_from": tuned_from, "tuning_region_blocks/digit": tuning,
         "workgroups_per_cu": group.workgroups_per_cu,
         "keygen_s": round(t_gen, 2),
         "checks": {"all_found": all_found, "exact_counts_16_filters": exact,

@@ -533,6 +533,16 @@ extern "C" int gqf_autotune(int nbits, double load, uint32_t* region_blocks, uin
   return 0;
 }
 
+extern "C" int gqf_autotune_set(int nbits, double load, uint32_t region_blocks, uint32_t sort_digit_bits) {
+  if (nbits < 8 || nbits > 32 || !(load > 0) || load > 1.0) return -2;
+  if (region_blocks < GQF_REGION_BLOCKS_MIN || region_blocks > GQF_REGION_BLOCKS_MAX) return -2;
+  if (sort_digit_bits && (sort_digit_bits < 6 || sort_digit_bits > 10)) return -2;  // as qf_set_tuning
+  const int bucket = (int)lround(load * 20.0);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[{nbits, bucket}] = {region_blocks, sort_digit_bits};
+  return 0;
+}
+
 namespace {
 
 // Workgroups per CU of the group's persistent rebuild: times the group

@@ -38,7 +38,7 @@ EXPORTS = [
     "qf_reserve", "qf_set_tuning", "qf_reset_device", "bulk_insert_async", "bulk_insert_reduce_async", "bulk_delete_async",
     "bulk_get_async", "qf_last_error", "gqf_profile_enable", "gqf_profile_read", "gqf_fill_splitmix64",
     "gqf_group_create", "gqf_group_destroy", "gqf_group_bulk_insert", "gqf_group_bulk_get",
-    "gqf_group_set_graphs", "gqf_autotune", "gqf_group_autotune",
+    "gqf_group_set_graphs", "gqf_autotune", "gqf_autotune_set", "gqf_group_autotune",
     "gqf_group_set_workgroups_per_cu", "gqf_group_get_workgroups_per_cu", "gqf_group_set_wait_bound",
     "qf_batch_pieces", "bulk_insert_noalloc", "qf_check_layout",
     "qf_server_create", "qf_server_add_filter", "qf_server_submit", "qf_server_run", "qf_server_wait",
@@ -126,6 +126,8 @@ def load_library(path=LIB_PATH):
     L.gqf_autotune.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32),
                                ctypes.POINTER(ctypes.c_uint32)]
     L.gqf_autotune.restype = ctypes.c_int
+    L.gqf_autotune_set.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32]
+    L.gqf_autotune_set.restype = ctypes.c_int
     L.gqf_group_autotune.argtypes = [_vp, ctypes.POINTER(_u64)]
     L.gqf_group_autotune.restype = ctypes.c_int
     L.gqf_group_set_workgroups_per_cu.argtypes = [_vp, ctypes.c_uint32]

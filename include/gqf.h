@@ -202,6 +202,12 @@ int gqf_group_set_graphs(gqf_group *g, int on);
  * in one batch for each candidate region size and sort digit, returns the
  * fastest (cached per nbits and load rounded to 0.05); synchronises */
 int gqf_autotune(int nbits, double load, uint32_t *region_blocks, uint32_t *sort_digit_bits);
+/* records a choice in the autotuner's cache (as a measurement would): a
+ * table of choices measured once and committed (cqf_amd/tuning_gfx950.json)
+ * is loaded this way, so later gqf_autotune calls for those sizes time
+ * nothing.  -2 for nbits outside 8..32, load outside (0, 1], a region size
+ * outside 16..56 or a digit width other than 0 (default) or 6..10 */
+int gqf_autotune_set(int nbits, double load, uint32_t region_blocks, uint32_t sort_digit_bits);
 /* gqf_autotune for every filter of the group at the load its segment
  * brings, applied with qf_set_tuning; then the workgroups per CU of the
  * group's persistent rebuild, timed for a batch cut like `offsets` on

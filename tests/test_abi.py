@@ -65,3 +65,33 @@ def test_drop_in_caller_library_loads():
     lib = ctypes.CDLL(B.LIB)
     for n in ("dropin_gqf_init", "dropin_point_bulk_insert", "dropin_count_kmers", "dropin_gqf_bulk_get"):
         assert hasattr(lib, n)
+
+
+def test_autotune_set_validates_and_table_is_well_formed():
+    """gqf_autotune_set takes the choices a measurement could make and
+    rejects the rest (host cache only, no GPU call); the committed table of
+    choices, when present, holds only such entries."""
+    from cqf_amd import gqf, tuning
+    L = gqf.load_library()
+    assert L.gqf_autotune_set(20, 0.9, 48, 0) == 0
+    assert L.gqf_autotune_set(20, 0.9, 40, 8) == 0
+    assert L.gqf_autotune_set(7, 0.9, 48, 0) == -2     # nbits below 8
+    assert L.gqf_autotune_set(20, 0.0, 48, 0) == -2    # load not in (0, 1]
+    assert L.gqf_autotune_set(20, 1.5, 48, 0) == -2
+    assert L.gqf_autotune_set(20, 0.9, 12, 0) == -2    # region size outside 16..56
+    assert L.gqf_autotune_set(20, 0.9, 64, 0) == -2
+    assert L.gqf_autotune_set(20, 0.9, 48, 5) == -2    # digit width other than 0 or 6..10
+    if os.path.exists(tuning.TABLE):
+        choices = tuning.read()
+        assert choices
+        for (q, ld), (r, d) in choices.items():
+            assert 8 <= q <= 32 and 0 < ld <= 1.0
+            assert 16 <= r <= 56 and (d == 0 or 6 <= d <= 10)
+
+def test_tuning_table_round_trip(tmp_path):
+    """A table written by cqf_amd.tuning reads back to the same choices."""
+    from cqf_amd import tuning
+    choices = {(16, 0.5): (48, 0), (20, 0.95): (40, 8), (28, 0.9): (48, 0)}
+    path = str(tmp_path / "t.json")
+    tuning.write(choices, path, "test")
+    assert tuning.read(path) == choices

@@ -3,6 +3,8 @@ rebuild launch (captured into a hipGraph), queried by one launch.  Each
 filter must end exactly as if its segment had gone through bulk_insert
 alone: the oracle's image, counts and hashed keys.  Every test prints its
 case before it touches the GPU."""
+import time
+
 import numpy as np
 import pytest
 import torch
@@ -155,6 +157,12 @@ def test_autotune_choices_keep_results(gpu, oracle):
     _say("case autotune q=14 load 0.9")
     r, d = gqf.autotune(14, 0.9)
     assert 16 <= r <= 56 and d in (0, 6, 7, 8, 9, 10)
+    # a choice recorded with gqf_autotune_set (as the committed table is
+    # loaded) is returned without timing anything
+    assert gqf.load_library().gqf_autotune_set(15, 0.55, 40, 8) == 0
+    t0 = time.perf_counter()
+    assert gqf.autotune(15, 0.55) == (40, 8)
+    assert time.perf_counter() - t0 < 0.05
     params, keys, offsets, filters = _setup(4, 12, 14, seed=12)
     group = gqf.GQFGroup(filters)
     try:
