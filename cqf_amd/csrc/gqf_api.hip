@@ -56,11 +56,11 @@ struct Workspace {
   unsigned long long* misses = nullptr;
   // counting reduce (allocated on first use): hash table, counts by
   // compaction index, scan tile sums, control words
-  uint64_t* agg_keys = nullptr;
-  uint64_t* agg_cnt = nullptr;
+  uint64_t* agg_stage = nullptr;   // GQF_AGG_BUCKETS x GQF_AGG_BUCKET_CAP staged (hash, count) entries
   uint64_t* agg_by_idx = nullptr;
   uint64_t* agg_tiles = nullptr;
-  uint32_t* agg_ctrl = nullptr;
+  uint64_t* agg_hot = nullptr;
+  uint32_t* agg_ctrl = nullptr;    // 4 control words, then the bucket fills
 };
 
 struct Filter {
@@ -145,16 +145,18 @@ void ensure_workspace(Filter* f, uint64_t n) {
 }
 
 // Counting reduce of a duplicate-heavy batch of hashed keys: the hashes are
-// counted in a table and only the distinct ones sorted (gqf_agg.hip), which
-// leaves `keys` and the (hash, count) pairs exactly as the sort and the
-// run-length reduce would.  Batches of at least 2^22 keys whose sample of
-// 4096 holds at most 2048 distinct hashes try it; a batch with more distinct
-// hashes than the table holds (3/4 of 2^22) falls back to the sort.  Reads
-// two control words back, so it synchronises the stream twice (the
-// reference's bulk_insert_reduce synchronises as well, src/gqf.cu:3247-3250,
-// and reads reduce_by_key's end back, :3273).  Returns false when not taken
-// (nothing in `keys` or the table changed); else *m_out = distinct hashes.
-// A stream under graph capture keeps the sort path (no host reads there).
+// counted per tile, staged by bucket and summed per bucket, and only the
+// distinct ones sorted (gqf_agg.hip), which leaves `keys` and the (hash,
+// count) pairs exactly as the sort and the run-length reduce would.
+// Batches of at least 2^22 keys whose sample of 4096 holds at most 2048
+// distinct keys try it; a batch whose distinct hashes do not fit (a staging
+// bucket, a bucket's LDS table or the 2^22 output entries) falls back to the
+// sort.  Reads the control words back, so it synchronises the stream twice
+// (the reference's bulk_insert_reduce synchronises as well,
+// src/gqf.cu:3247-3250, and reads reduce_by_key's end back, :3273).
+// Returns false when not taken (nothing in `keys` changed); else *m_out =
+// distinct hashes.  A stream under graph capture keeps the sort path (no
+// host reads there).
 constexpr uint64_t kAggMinKeys = 1ULL << 22;
 
 bool stream_capturing(hipStream_t s) {
@@ -169,33 +171,32 @@ bool reduce_by_counting(Filter* f, uint64_t n, uint64_t* keys, int bits, hipStre
   const uint64_t cap = 1ULL << GQF_AGG_LOG_CAP;
   {
     std::lock_guard<std::mutex> lk(f->mu);
-    if (!w.agg_keys) {
-      dev_alloc(&w.agg_keys, cap);
-      dev_alloc(&w.agg_cnt, cap);
+    if (!w.agg_stage) {
+      dev_alloc(&w.agg_stage, (uint64_t)GQF_AGG_BUCKETS * GQF_AGG_BUCKET_CAP);
       dev_alloc(&w.agg_by_idx, cap);
       dev_alloc(&w.agg_tiles, 4096);
-      dev_alloc(&w.agg_ctrl, 4);
-      if (!w.agg_keys || !w.agg_cnt || !w.agg_by_idx || !w.agg_tiles || !w.agg_ctrl) return false;
-      gqf_launch_agg_clear(w.agg_keys, w.agg_cnt, cap, s);
+      dev_alloc(&w.agg_hot, GQF_AGG_HOT + 1);
+      dev_alloc(&w.agg_ctrl, 4 + GQF_AGG_BUCKETS);
+      if (!w.agg_stage || !w.agg_by_idx || !w.agg_tiles || !w.agg_hot || !w.agg_ctrl) return false;
     }
   }
   const uint64_t mask = gqf_mask(f->host.key_bits);
   const int vb = (int)f->host.value_bits;
+  uint32_t* fill = w.agg_ctrl + 4;
   uint32_t ctrl[4] = {0, 0, 0, 0};
-  gqf_launch_agg_sample(keys, n, mask, vb, w.agg_ctrl, s);
+  gqf_launch_agg_sample(keys, n, mask, w.agg_ctrl, fill, w.agg_hot, s);
   GQF_CHECK(hipMemcpyAsync(ctrl, w.agg_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
   GQF_CHECK(hipStreamSynchronize(s));
   if (ctrl[0] > GQF_AGG_SAMPLE / 2) return false;
-  gqf_launch_agg_tiles(keys, n, mask, vb, w.agg_keys, w.agg_cnt, GQF_AGG_LOG_CAP, w.agg_ctrl, s);
+  gqf_launch_agg_tiles(keys, n, mask, vb, w.agg_hot, w.agg_stage, fill, w.agg_ctrl, s);
+  // comp goes to upos (free until the finish's prefix sums overwrite it)
+  gqf_launch_agg_merge(w.agg_stage, fill, width, w.upos, w.agg_by_idx, std::min(cap, w.cap), w.agg_ctrl, s);
   GQF_CHECK(hipMemcpyAsync(ctrl, w.agg_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
   GQF_CHECK(hipStreamSynchronize(s));
   const uint64_t m = ctrl[1];
-  if (ctrl[2] != 0 || m > cap / 4 * 3 || m == 0) {
-    gqf_launch_agg_clear(w.agg_keys, w.agg_cnt, cap, s);
-    return false;
-  }
-  gqf_launch_agg_finish(keys, n, m, width, w.agg_keys, w.agg_cnt, cap, w.agg_ctrl, w.upos, w.agg_by_idx, w.tmp,
-                        w.hist, w.totals, w.agg_tiles, w.tile_heads, w.ukeys, w.ucounts, w.upos, s);
+  if (ctrl[2] != 0 || m > cap || m == 0) return false;
+  gqf_launch_agg_finish(keys, n, m, width, w.upos, w.agg_by_idx, w.tmp, w.hist, w.totals, w.agg_tiles, w.tile_heads,
+                        w.ukeys, w.ucounts, w.upos, s);
   *m_out = m;
   return true;
 }
@@ -427,10 +428,10 @@ void free_all_workspace(Workspace& w) {
   dev_free(w.wide_flags);
   dev_free(w.wide_count);
   dev_free(w.misses);
-  dev_free(w.agg_keys);
-  dev_free(w.agg_cnt);
+  dev_free(w.agg_stage);
   dev_free(w.agg_by_idx);
   dev_free(w.agg_tiles);
+  dev_free(w.agg_hot);
   dev_free(w.agg_ctrl);
 }
 

@@ -39,27 +39,36 @@ void gqf_launch_sort_two_level(uint64_t* keys, uint64_t* tmp, uint64_t n, uint64
                                uint32_t* hist, uint32_t* totals, uint32_t* slow, hipStream_t s,
                                const gqf_region_args* regions = nullptr, uint64_t tag = 0);
 
-// Counting reduce of duplicate-heavy batches (gqf_agg.hip).  ctrl: 4 words.
-// sample: ctrl[0] = distinct hashes among 4096 keys spread over the batch.
+// Counting reduce of duplicate-heavy batches (gqf_agg.hip).  ctrl: 4 words
+// (distinct keys in the sample, distinct hashes, overflow flag, output
+// cursor); fill: GQF_AGG_BUCKETS words; hot: GQF_AGG_HOT + 1 words; stage:
+// GQF_AGG_BUCKETS x GQF_AGG_BUCKET_CAP words.
 #define GQF_AGG_SAMPLE 4096
-#define GQF_AGG_LOG_CAP 22  // global table of 2^22 (hash, count) entries
-void gqf_launch_agg_sample(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, uint32_t* ctrl,
-                           hipStream_t s);
-// counts every key's slot hash into the table (gkeys / gcnt, 2^log_cap
-// entries, empty on entry): ctrl[1] = distinct hashes, ctrl[2] != 0 if the
-// table ran out of room (then clear it with gqf_launch_agg_clear)
-void gqf_launch_agg_tiles(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, uint64_t* gkeys,
-                          uint64_t* gcnt, int log_cap, uint32_t* ctrl, hipStream_t s);
-void gqf_launch_agg_clear(uint64_t* gkeys, uint64_t* gcnt, uint64_t cap, hipStream_t s);
-// the table's m entries -> ascending ukeys / ucounts (m each), upos (m + 1
-// prefix sums) and keys = each hash count times in ascending order; empties
-// the table.  width = gqf_sort_width(hash bits, 0), width + log_cap <= 64;
-// comp: m words; cnt: 2^log_cap words; tmp / hist / totals: a sort of m
-// keys; tile_sums: 4096 words; starts: n / 4096 + 2 words.
-void gqf_launch_agg_finish(uint64_t* keys, uint64_t n, uint64_t m, int width, uint64_t* gkeys, uint64_t* gcnt,
-                           uint64_t cap, uint32_t* ctrl, uint64_t* comp, uint64_t* cnt, uint64_t* tmp, uint32_t* hist,
-                           uint32_t* totals, uint64_t* tile_sums, uint32_t* starts, uint64_t* ukeys,
-                           uint64_t* ucounts, uint64_t* upos, hipStream_t s);
+#define GQF_AGG_HOT 8                // keys counted in registers (the sample's most frequent)
+#define GQF_AGG_BUCKETS 512          // staging buckets, by hash
+#define GQF_AGG_BUCKET_CAP 32768     // staging entries per bucket (2^24 in all)
+#define GQF_AGG_LOG_CAP 22           // distinct hashes a counted batch may hold
+// sample: ctrl[0] = distinct keys among 4096 spread over the batch, the hot
+// keys, ctrl[1..3] and fill zeroed
+void gqf_launch_agg_sample(const uint64_t* keys, uint64_t n, uint64_t mask, uint32_t* ctrl, uint32_t* fill,
+                           uint64_t* hot, hipStream_t s);
+// counts every key's slot hash per 131072-key tile and stages the tiles'
+// (hash, count) entries by bucket; ctrl[2] != 0 if a bucket ran out of room
+void gqf_launch_agg_tiles(const uint64_t* keys, uint64_t n, uint64_t mask, int vshift, const uint64_t* hot,
+                          uint64_t* stage, uint32_t* fill, uint32_t* ctrl, hipStream_t s);
+// sums the staged entries by hash: comp[idx] = (idx << width) | hash,
+// cnt[idx] = count for the ctrl[1] distinct hashes (ctrl[2] != 0 if they
+// did not fit: LDS per bucket or cnt_cap)
+void gqf_launch_agg_merge(const uint64_t* stage, const uint32_t* fill, int width, uint64_t* comp, uint64_t* cnt,
+                          uint64_t cnt_cap, uint32_t* ctrl, hipStream_t s);
+// the m merged entries -> ascending ukeys / ucounts (m each), upos (m + 1
+// prefix sums) and keys = each hash count times in ascending order.
+// width = gqf_sort_width(hash bits, 0); comp: m words (may be upos);
+// tmp / hist / totals: a sort of m keys; tile_sums: 4096 words; starts:
+// n / 4096 + 2 words.
+void gqf_launch_agg_finish(uint64_t* keys, uint64_t n, uint64_t m, int width, uint64_t* comp, uint64_t* cnt,
+                           uint64_t* tmp, uint32_t* hist, uint32_t* totals, uint64_t* tile_sums, uint32_t* starts,
+                           uint64_t* ukeys, uint64_t* ucounts, uint64_t* upos, hipStream_t s);
 
 // run-length reduce of sorted keys -> ukeys/ucounts, count -> *m_dev.
 // tile_heads: ceil(n/4096) words; upos: n words.
