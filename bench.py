@@ -297,8 +297,9 @@ def zipf_main(args):
         step(recs)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern = {k: gqf.profile_read(k) for k in ("reduce_count", "reduce_finish", "sort_pass", "sort_buckets",
-                                              "region_bounds", "rebuild_phase", "rebuild_wide", "bulk_get")}
+    kern = {k: gqf.profile_read(k) for k in ("reduce_count", "reduce_merge", "reduce_finish", "sort_pass",
+                                              "sort_buckets", "region_bounds", "rebuild_phase", "rebuild_wide",
+                                              "bulk_get")}
     gqf.profile_enable(False)
     ins = [e[0].elapsed_time(e[1]) for e in recs]
     qry = [e[1].elapsed_time(e[2]) for e in recs]
