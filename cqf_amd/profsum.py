@@ -1,7 +1,7 @@
 """Summaries of rocprofv3 CSV output (kernel stats and counter collection).
 
     python -m cqf_amd.profsum stats  <run>_kernel_stats.csv
-    python -m cqf_amd.profsum counters <run>_counter_collection.csv [--by-grid]
+    python -m cqf_amd.profsum counters <run>_counter_collection.csv [--by-grid] [--first N]
     python -m cqf_amd.profsum trace <run>_kernel_trace.csv
 
 `stats` prints calls / total / average duration per kernel; `counters`
@@ -30,10 +30,20 @@ def stats(path):
               f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
 
 
-def counters(path, by_grid=False):
+def counters(path, by_grid=False, first=0):
+    """`first` > 0: only each kernel's first `first` dispatches (in dispatch
+    order), e.g. the insert launches of a run whose deletes follow."""
     acc = defaultdict(lambda: [0, 0.0])
-    for r in csv.DictReader(open(path)):
-        k = (r["Kernel_Name"].strip(), r["Counter_Name"].strip(), r["Grid_Size"] if by_grid else "")
+    seen = defaultdict(list)  # kernel -> dispatch ids in order
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+    for r in rows:
+        kern = r["Kernel_Name"].strip()
+        d = int(r["Dispatch_Id"])
+        if d not in seen[kern]:
+            seen[kern].append(d)
+        if first and len(seen[kern]) > first:
+            continue
+        k = (kern, r["Counter_Name"].strip(), r["Grid_Size"] if by_grid else "")
         acc[k][0] += 1
         acc[k][1] += float(r["Counter_Value"])
     print("| kernel | grid | counter | dispatches | mean per dispatch |")
@@ -56,6 +66,8 @@ def trace(path):
 if __name__ == "__main__":
     cmd, path = sys.argv[1], sys.argv[2]
     if cmd == "counters":
-        counters(path, by_grid="--by-grid" in sys.argv[3:])
+        rest = sys.argv[3:]
+        first = int(rest[rest.index("--first") + 1]) if "--first" in rest else 0
+        counters(path, by_grid="--by-grid" in rest, first=first)
     else:
         {"stats": stats, "trace": trace}[cmd](path)
