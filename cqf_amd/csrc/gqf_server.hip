@@ -8,8 +8,8 @@
 //     done: the time is recorded);
 //   * for every filter with queued requests and fewer than max_inflight
 //     batches on the device, admits the filter if it is not resident
-//     (evicting least recently used idle filters to host images until it
-//     fits the HBM budget) and launches the head of its FIFO -- requests of
+//     (evicting least recently used idle filters to pinned host copies of
+//     their device blocks until it fits the HBM budget) and launches the head of its FIFO -- requests of
 //     every kind, in arrival order, up to max_batch_keys keys and
 //     QF_SEQ_MAX_REQUESTS requests per side -- as ONE sequenced batch on
 //     the filter's stream:
@@ -81,8 +81,10 @@ struct SFilter {
   uint32_t fid = 0;
   int nbits = 0;
   QF* qf = nullptr;
-  std::vector<uint8_t> image;  // host image while evicted
-  bool has_image = false;
+  uint8_t* host_blocks = nullptr;  // pinned copy of the device blocks while evicted
+  uint64_t host_bytes = 0;
+  uint32_t host_status = 0;
+  bool evicted = false;
   std::deque<int64_t> fifo;
   std::deque<Batch> inflight;
   uint64_t last_use = 0;
@@ -171,12 +173,24 @@ bool evict_one(qf_server* s) {
     if (f.qf && f.fifo.empty() && f.inflight.empty() && (!v || f.last_use < v->last_use)) v = &f;
   }
   if (!v) return false;
-  gqf_info info;
-  if (qf_get_info(v->qf, &info) != 0) return false;
+  // the device blocks as they are (no format change) into a pinned host
+  // buffer kept for the filter's next eviction, and the status word
+  QF d;
+  if (!gqf_filter_host(v->qf, &d)) return false;
+  const uint64_t bytes = d.nblocks * GQF_BLOCK_BYTES;
+  if (v->host_bytes < bytes) {
+    if (v->host_blocks) (void)hipHostFree(v->host_blocks);
+    v->host_blocks = nullptr;
+    v->host_bytes = 0;
+    check(s, hipHostMalloc((void**)&v->host_blocks, bytes, hipHostMallocDefault), "hipHostMalloc");
+    if (!v->host_blocks) return false;
+    v->host_bytes = bytes;
+  }
+  check(s, hipMemcpyAsync(v->host_blocks, d.blocks, bytes, hipMemcpyDeviceToHost, v->stream), "hipMemcpyAsync");
+  check(s, hipMemcpyAsync(&v->host_status, d.status, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream),
+        "hipMemcpyAsync");
   check(s, hipStreamSynchronize(v->stream), "hipStreamSynchronize");
-  v->image.resize(info.total_size_in_bytes);
-  if (qf_export_image(v->qf, v->image.data(), v->image.size()) != 0) return false;
-  v->has_image = true;
+  v->evicted = true;
   qf_destroy_device(v->qf);
   v->qf = nullptr;
   release_buffers(*v);
@@ -195,11 +209,19 @@ bool admit(qf_server* s, SFilter& f) {
   check(s, hipMalloc(&f.gbuf, s->max_batch * 8), "hipMalloc");
   check(s, hipMalloc(&f.mbuf, s->max_batch * 8), "hipMalloc");
   check(s, hipMalloc(&f.cbuf, s->max_batch * 8), "hipMalloc");
-  if (f.has_image) {
-    if (qf_import_image(f.qf, f.image.data(), f.image.size()) != 0) s->err = -4;
-    f.image.clear();
-    f.image.shrink_to_fit();
-    f.has_image = false;
+  if (f.evicted) {
+    QF d;
+    if (!gqf_filter_host(f.qf, &d) || d.nblocks * GQF_BLOCK_BYTES > f.host_bytes) {
+      s->err = -4;
+    } else {
+      check(s, hipMemcpyAsync(d.blocks, f.host_blocks, d.nblocks * GQF_BLOCK_BYTES, hipMemcpyHostToDevice, f.stream),
+            "hipMemcpyAsync");
+      check(s, hipMemcpyAsync(d.status, &f.host_status, sizeof(uint32_t), hipMemcpyHostToDevice, f.stream),
+            "hipMemcpyAsync");
+      // the host words are read by the copies: wait before they can change
+      check(s, hipStreamSynchronize(f.stream), "hipStreamSynchronize");
+    }
+    f.evicted = false;
   }
   s->used += need;
   s->stats.admissions++;
@@ -505,6 +527,7 @@ extern "C" void qf_server_destroy(qf_server* s) {
     if (f.qf) qf_destroy_device(f.qf);
     release_buffers(f);
     if (f.desc) (void)hipHostFree(f.desc);
+    if (f.host_blocks) (void)hipHostFree(f.host_blocks);
     for (Batch& b : f.inflight) (void)hipEventDestroy(b.ev);
   }
   for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
