@@ -239,8 +239,8 @@ void run_mutation(Filter* f, Workspace& w, uint64_t n, uint64_t* keys, uint8_t f
   if (reduce && hash_it && reduce_by_counting(f, n, keys, bits, s, &m)) {
     gqf_launch_region_bounds(f->dev, f->host.nregions, w.ukeys, m, nullptr, w.region_start, w.region_list,
                              w.list_count, ~0ULL, s);
-    gqf_launch_rebuild(f->dev, f->host.nregions, w.ukeys, w.ucounts, m, w.region_start, w.region_list, w.list_count,
-                       mode, w.wide_flags, w.wide_count, s);
+    gqf_launch_rebuild(f->dev, f->host.nregions, f->host.region_blocks, w.ukeys, w.ucounts, m, w.region_start,
+                       w.region_list, w.list_count, mode, w.wide_flags, w.wide_count, s);
     GQF_CHECK(hipGetLastError());
     return;
   }
@@ -283,8 +283,8 @@ void run_mutation(Filter* f, Workspace& w, uint64_t n, uint64_t* keys, uint8_t f
   else
     gqf_launch_region_bounds(f->dev, f->host.nregions, items, n, n_dev, w.region_start, w.region_list,
                              w.list_count, ~0ULL, s);
-  gqf_launch_rebuild(f->dev, f->host.nregions, items, counts, n, w.region_start, w.region_list, w.list_count, mode,
-                     w.wide_flags, w.wide_count, s);
+  gqf_launch_rebuild(f->dev, f->host.nregions, f->host.region_blocks, items, counts, n, w.region_start,
+                     w.region_list, w.list_count, mode, w.wide_flags, w.wide_count, s);
   GQF_CHECK(hipGetLastError());
 }
 
@@ -718,8 +718,8 @@ extern "C" int qf_bulk_mutate_seq(QF* qf, uint64_t n, uint64_t* keys, uint32_t n
                   w.totals, s);
   gqf_launch_region_bounds(f->dev, f->host.nregions, keys, n, nullptr, w.region_start, w.region_list, w.list_count,
                            GQF_SEQ_HASH_MASK, s);
-  gqf_launch_rebuild(f->dev, f->host.nregions, keys, nullptr, n, w.region_start, w.region_list, w.list_count,
-                     GQF_REBUILD_SEQ, w.wide_flags, w.wide_count, s);
+  gqf_launch_rebuild(f->dev, f->host.nregions, f->host.region_blocks, keys, nullptr, n, w.region_start,
+                     w.region_list, w.list_count, GQF_REBUILD_SEQ, w.wide_flags, w.wide_count, s);
   GQF_CHECK(hipGetLastError());
   return 0;
 }
@@ -1105,7 +1105,7 @@ int64_t move_items(const QF* src_dev, const QF& src, const QF* dst_dev, const QF
                            gqf_mask(src.key_bits), items, counts, nullptr, cap, s);
       gqf_launch_region_bounds(dst_dev, dst.nregions, items, n, nullptr, dst_ws.region_start, dst_ws.region_list,
                                dst_ws.list_count, ~0ULL, s);
-      gqf_launch_rebuild(dst_dev, dst.nregions, items, counts, n, dst_ws.region_start, dst_ws.region_list,
+      gqf_launch_rebuild(dst_dev, dst.nregions, dst.region_blocks, items, counts, n, dst_ws.region_start, dst_ws.region_list,
                          dst_ws.list_count, kModeInsert, dst_ws.wide_flags, dst_ws.wide_count, s);
     }
     u = v;

@@ -126,10 +126,10 @@ void gqf_launch_seq_adjust(const QF* qf_dev, const uint64_t* keys, uint64_t n, c
 // kernels leave them so)
 #define GQF_WIDE_LIST 64
 #define GQF_WIDE_WORDS (2 + GQF_WIDE_LIST)
-void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* items, const uint64_t* counts,
-                        uint64_t n_items, const uint32_t* region_start, const uint32_t* region_list,
-                        const uint32_t* list_count, int mode, uint8_t* wide_flags, uint32_t* wide_count,
-                        hipStream_t s);
+void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blocks, const uint64_t* items,
+                        const uint64_t* counts, uint64_t n_items, const uint32_t* region_start,
+                        const uint32_t* region_list, const uint32_t* list_count, int mode, uint8_t* wide_flags,
+                        uint32_t* wide_count, hipStream_t s);
 
 void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uint64_t* returns, hipStream_t s);
 void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsigned long long* misses,

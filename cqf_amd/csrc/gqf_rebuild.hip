@@ -42,6 +42,7 @@ namespace {
 constexpr int kWinFast = 64;          // region blocks + push blocks
 constexpr int kWinMid = 128;          // first fallback window (80 push blocks)
 constexpr int kWinWide = 256;         // last fallback window (208 push blocks)
+constexpr int kWideClassGrid = 256;  // workgroups of a class launch of the wide pass
 constexpr int kItemCap = 256;         // items cached in LDS per region (fast path): 192 +- 14 at 2^24 keys into 2^28 slots
 constexpr int32_t kNegInf = INT32_MIN / 4;
 
@@ -1490,20 +1491,65 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
         }
       }
     }
-  } else if (blockIdx.x == 0) {
-    for (uint64_t base = (uint64_t)parity; base < qf.nregions; base += 128) {
-      const uint64_t r = base + 2ULL * lane;
-      const bool f = r < qf.nregions && wide_flags[r];
-      uint64_t m = __ballot(f);
-      while (m) {
-        const int l = __builtin_ctzll(m);
-        m &= m - 1;
-        redo(base + 2ULL * l);
-      }
-    }
+  } else {
+    return;  // more than GQF_WIDE_LIST flagged: the class launches take them (gqf_rebuild_wide_class)
   }
   // the last workgroup to finish clears the count and the finish counter
   if (lane == 0) {
+    __threadfence();
+    if (atomicAdd(&wide_count[1 + GQF_WIDE_LIST], 1u) == gridDim.x - 1) {
+      wide_count[1 + GQF_WIDE_LIST] = 0;
+      *(volatile uint32_t*)wide_count = 0;
+    }
+  }
+}
+
+// The flagged regions of a phase when there are more than GQF_WIDE_LIST:
+// regions of one parity whose indices differ by at least `apart` (the
+// widest window, 256 blocks, in regions) have disjoint windows, so they
+// can be redone at once.  Region parity + 2 i is in class i mod ncls
+// (ncls = ceil(apart / 2)); the launcher runs the classes one after
+// another, each over the whole grid, and the last class's last workgroup
+// clears the count.  Nothing runs when GQF_WIDE_LIST or fewer were
+// flagged (gqf_rebuild_wide has taken them).
+extern "C" __global__ void __launch_bounds__(64)
+gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
+                       const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
+                       int parity, int mode, uint8_t* __restrict__ wide_flags, uint32_t* __restrict__ wide_count,
+                       uint32_t cls, uint32_t ncls) {
+  __shared__ Smem<kWinMid> smm;
+  __shared__ Smem<kWinWide> sm;
+  __shared__ uint32_t icnt[kItemCap];
+  const uint32_t nflag = *(volatile uint32_t*)wide_count;
+  if (nflag <= GQF_WIDE_LIST) return;
+  const QF qf = *qfp;
+  const int lane = lane_id();
+  const uint64_t nper = qf.nregions > (uint64_t)parity ? (qf.nregions - (uint64_t)parity + 1) / 2 : 0;
+  // candidate j of this class is i = cls + ncls j, region parity + 2 i;
+  // workgroups take 64 candidates at a time
+  const uint64_t ncand = nper > cls ? (nper - cls + ncls - 1) / ncls : 0;
+  for (uint64_t j0 = (uint64_t)blockIdx.x * 64; j0 < ncand; j0 += (uint64_t)gridDim.x * 64) {
+    const uint64_t j = j0 + (uint64_t)lane;
+    const uint64_t r = (uint64_t)parity + 2 * ((uint64_t)cls + (uint64_t)ncls * j);
+    uint64_t m = __ballot(j < ncand && wide_flags[r]);
+    while (m) {
+      const int l = __builtin_ctzll(m);
+      m &= m - 1;
+      const uint64_t rr = (uint64_t)parity + 2 * ((uint64_t)cls + (uint64_t)ncls * (j0 + (uint64_t)l));
+      const uint64_t s = region_start[rr], e = region_start[rr + 1];
+      int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt, false);
+      __syncthreads();
+      if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt, false);
+      __syncthreads();
+      if (lane == 0) {
+        wide_flags[rr] = 0;
+        if (res == kNeedWide) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
+        else if (res == kNoSpace) atomicOr(qf.status, GQF_STATUS_NO_SPACE);
+      }
+      __syncthreads();
+    }
+  }
+  if (cls + 1 == ncls && lane == 0) {
     __threadfence();
     if (atomicAdd(&wide_count[1 + GQF_WIDE_LIST], 1u) == gridDim.x - 1) {
       wide_count[1 + GQF_WIDE_LIST] = 0;
@@ -1809,10 +1855,10 @@ void gqf_launch_region_lists(const QF* qf_dev, uint64_t nregions, uint64_t n, ui
   gqf_prof_end("region_bounds", s);
 }
 
-void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* items, const uint64_t* counts,
-                        uint64_t n_items, const uint32_t* region_start, const uint32_t* region_list,
-                        const uint32_t* list_count, int mode, uint8_t* wide_flags, uint32_t* wide_count,
-                        hipStream_t s) {
+void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blocks, const uint64_t* items,
+                        const uint64_t* counts, uint64_t n_items, const uint32_t* region_start,
+                        const uint32_t* region_list, const uint32_t* list_count, int mode, uint8_t* wide_flags,
+                        uint32_t* wide_count, hipStream_t s) {
   // at most one workgroup per non-empty region: no more than the regions
   // of a parity, nor than the items
   const uint64_t per_parity = (nregions + 1) / 2;
@@ -1831,6 +1877,13 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, const uint64_t* ite
     gqf_prof_begin("rebuild_wide", s);
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(GQF_WIDE_LIST), dim3(64), 0, s, qf_dev, items, counts, region_start,
                        parity, mode, wide_flags, wide_count);
+    // more than GQF_WIDE_LIST flagged (loads near 95%): the classes of
+    // regions whose widest windows are disjoint, one launch each
+    const uint32_t apart = (uint32_t)((kWinWide + region_blocks - 1) / region_blocks);
+    const uint32_t ncls = (apart + 1) / 2;
+    for (uint32_t c = 0; c < ncls; c++)
+      hipLaunchKernelGGL(gqf_rebuild_wide_class, dim3(kWideClassGrid), dim3(64), 0, s, qf_dev, items, counts,
+                         region_start, parity, mode, wide_flags, wide_count, c, ncls);
     gqf_prof_end("rebuild_wide", s);
   }
 }
