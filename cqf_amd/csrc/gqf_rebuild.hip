@@ -138,6 +138,69 @@ struct Items {
   }
 };
 
+// The straight-line insert merge (one or two items with distinct hashes
+// on a run of at most 8 count-1 counters, see merge_quotient), split in
+// two so a caller can keep what the reading half finds in registers until
+// the writing half: lo / hi = the old run's bytes, meta = 1 << 31 |
+// old_len << 20 | n << 18 | r1 << 8 | r0 (r1 = 0x200 when n = 1).  Bytes
+// past the run read as sentinels 0x100 + k (ascending, above every byte);
+// writes of absent bytes go to the lane's byte of the spare row.
+template <int W>
+__device__ __forceinline__ bool lean_read(const Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
+                                          int32_t st_o, uint32_t old_len, uint32_t h_first, uint32_t& lo,
+                                          uint32_t& hi, uint32_t& meta) {
+  const uint8_t* in = (const uint8_t*)sm.sin;
+  const uint32_t n = it1 - it0;
+  const uint32_t r0 = h_first & 0xFFu;
+  const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : 0x200u;
+  const uint32_t bo = sin_off(st_o);
+  const uint32_t ro = 64u - (uint32_t)(st_o & 63);  // bytes of the run's first row
+  uint32_t ob[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t b = in[bo + (uint32_t)k + ((uint32_t)k >= ro ? 4u : 0u)];
+    ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
+  }
+  bool simple = n == 1 || r1 > r0;
+#pragma unroll
+  for (int k = 1; k < 8; k++) simple = simple && ob[k] > ob[k - 1];
+  lo = (ob[0] & 0xFFu) | (ob[1] & 0xFFu) << 8 | (ob[2] & 0xFFu) << 16 | (ob[3] & 0xFFu) << 24;
+  hi = (ob[4] & 0xFFu) | (ob[5] & 0xFFu) << 8 | (ob[6] & 0xFFu) << 16 | (ob[7] & 0xFFu) << 24;
+  meta = 1u << 31 | old_len << 20 | n << 18 | r1 << 8 | r0;
+  return simple;
+}
+
+template <int W>
+__device__ __forceinline__ void lean_write(Smem<W>& sm, uint32_t lo, uint32_t hi, uint32_t meta, int32_t st_n) {
+  uint8_t* out = (uint8_t*)sm.sout;
+  const uint32_t old_len = (meta >> 20) & 0xFu, n = (meta >> 18) & 3u;
+  const uint32_t r0 = meta & 0xFFu, r1 = (meta >> 8) & 0x3FFu;
+  uint32_t ob[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t b = ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
+    ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
+  }
+  const uint32_t bn = sin_off(st_n);
+  const uint32_t rn = 64u - (uint32_t)(st_n & 63);
+  const uint32_t spare = (uint32_t)(W * 68) + (uint32_t)lane_id();
+  auto put_at = [&](uint32_t at, bool live, uint32_t v) {
+    const uint32_t a = bn + at + (at >= rn ? 4u : 0u);
+    out[live ? a : spare] = (uint8_t)v;
+  };
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    put_at((uint32_t)k + (r0 < ob[k] ? 1u : 0u) + (r1 < ob[k] ? 1u : 0u), (uint32_t)k < old_len, ob[k]);
+  uint32_t a0 = 0, a1 = 1;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    a0 += ob[k] <= r0 ? 1u : 0u;
+    a1 += ob[k] <= r1 ? 1u : 0u;
+  }
+  put_at(a0, true, r0);
+  put_at(a1, n == 2, r1);
+}
+
 // ---------------------------------------------------------------------------
 // Two-pointer merge of the old counters of one quotient (ascending
 // remainders, old run at st_o .. st_o + old_len - 1 of sin) with its items
@@ -166,42 +229,10 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
   // the run read as sentinels 0x100 + k (ascending, above every byte), and
   // writes of absent bytes go to the lane's byte of the spare row.
   if (src.kind == kAdd && it1 - it0 <= 2 && old_len <= 8) {
-    const uint32_t n = it1 - it0;
-    const uint32_t r0 = h_first & 0xFFu;
-    const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : 0x200u;
-    const uint32_t bo = sin_off(st_o);
-    const uint32_t ro = 64u - (uint32_t)(st_o & 63);  // bytes of the run's first row
-    uint32_t ob[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const uint32_t b = in[bo + (uint32_t)k + ((uint32_t)k >= ro ? 4u : 0u)];
-      ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
-    }
-    bool simple = n == 1 || r1 > r0;
-#pragma unroll
-    for (int k = 1; k < 8; k++) simple = simple && ob[k] > ob[k - 1];
-    if (simple) {
-      if (WRITE) {
-        const uint32_t bn = sin_off(st_n);
-        const uint32_t rn = 64u - (uint32_t)(st_n & 63);
-        const uint32_t spare = (uint32_t)(W * 68) + (uint32_t)lane_id();
-        auto put_at = [&](uint32_t at, bool live, uint32_t v) {
-          const uint32_t a = bn + at + (at >= rn ? 4u : 0u);
-          out[live ? a : spare] = (uint8_t)v;
-        };
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          put_at((uint32_t)k + (r0 < ob[k] ? 1u : 0u) + (r1 < ob[k] ? 1u : 0u), (uint32_t)k < old_len, ob[k]);
-        uint32_t a0 = 0, a1 = 1;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          a0 += ob[k] <= r0 ? 1u : 0u;
-          a1 += ob[k] <= r1 ? 1u : 0u;
-        }
-        put_at(a0, true, r0);
-        put_at(a1, n == 2, r1);
-      }
-      return old_len + n;
+    uint32_t lo, hi, meta;
+    if (lean_read(sm, src, it0, it1, st_o, old_len, h_first, lo, hi, meta)) {
+      if (WRITE) lean_write(sm, lo, hi, meta, st_n);
+      return old_len + (it1 - it0);
     }
   }
   // Common case: one item adding or removing 1 on a run (at most 8 slots) of
@@ -615,7 +646,7 @@ __device__ inline void or_bits(uint64_t* arr, int32_t d, uint64_t bits, int len)
   if (sh && sh + len > 64) atomicOr((unsigned long long*)&arr[w + 1], bits >> (64 - sh));
 }
 
-template <int W>
+template <int W, bool kStash>
 __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, int R, int nwin, const Items& src,
                                                     uint32_t m, Smem<W>& sm, uint64_t occ, uint64_t used,
                                                     uint64_t touched, bool table_end_in_window) {
@@ -625,16 +656,26 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
 
-  // ---- event position, old length and new length of every listed quotient
-  sm.xnew[j] = 0;
-  __syncthreads();
-  for (uint32_t t = lane; t < nq; t += 64) {
-    const int32_t q = (int32_t)(sm.items[sm.tit0[t]] >> 8);
+  // ---- event position, old length and new length of every listed quotient.
+  // With kStash, lane takes quotients lane + 64 k in round k (at most
+  // kItemCap = 4 x 64 of them), and a quotient that takes the straight-line
+  // merge keeps its old bytes and new remainders in registers for the
+  // writing pass (lean_* words; meta 0: the general merge).  Without it (a
+  // caller short of registers) every quotient takes the general merge.
+  static_assert(kItemCap <= 4 * 64, "four rounds of listed quotients");
+  uint32_t lean_lo[4], lean_hi[4], lean_meta[4] = {0, 0, 0, 0};
+  auto measure = [&](uint32_t t, int k) {
+    const uint32_t it0 = sm.tit0[t];
+    const uint32_t h0 = src.hash(sm.items, it0);
+    const int32_t q = (int32_t)(h0 >> 8);
     int32_t st = q;
     uint32_t ol = 0, nl = 0;
     if (locate_run_or_slot(sm, nwin, q, st, ol)) {
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st, ol, 0, src.hash(sm.items, sm.tit0[t]));
+      if (kStash && it1 - it0 <= 2 && ol <= 8 && lean_read(sm, src, it0, it1, st, ol, h0, lean_lo[k], lean_hi[k], lean_meta[k]))
+        nl = ol + (it1 - it0);
+      else
+        nl = merge_quotient<W, false>(sm, src, it0, it1, st, ol, 0, h0), lean_meta[k] = 0;
       atomicAdd((uint32_t*)&sm.xnew[st >> 6], 1u);  // events per slot-block (st < 64 nwin)
     } else {
       atomicMin(&sm.first_invalid, -1);
@@ -643,6 +684,15 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
     sm.tst[t] = (int16_t)st;
     sm.tol[t] = (uint16_t)ol;
     sm.tnl[t] = (uint16_t)nl;
+  };
+  sm.xnew[j] = 0;
+  __syncthreads();
+  if constexpr (kStash) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if ((uint32_t)lane + 64u * (uint32_t)k < nq) measure((uint32_t)lane + 64u * (uint32_t)k, k);
+  } else {
+    for (uint32_t t = lane; t < nq; t += 64) measure(t, 0);
   }
   __syncthreads();
   if (sm.first_invalid < 0) return kNeedWide;
@@ -910,17 +960,28 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   __syncthreads();
 
   // ---- the listed quotients' runs at their new starts
-  for (uint32_t t = lane; t < nq; t += 64) {
+  auto write_run = [&](uint32_t t, int k) {
     const uint32_t nl = sm.tnl[t];
     const int32_t sn = sm.tsn[t];
     const int32_t en = sn + (int32_t)nl - 1;
     if (sn < 0 || en >= 64 * bstar) {  // cannot happen (see above); no write leaves the rows
       atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
-      continue;
+      return;
     }
-    const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-    merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sn, src.hash(sm.items, sm.tit0[t]));
+    if (kStash && lean_meta[k]) {
+      lean_write(sm, lean_lo[k], lean_hi[k], lean_meta[k], sn);
+    } else {
+      const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
+      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sn, src.hash(sm.items, sm.tit0[t]));
+    }
     atomicOr((unsigned long long*)&sm.run_out[en >> 6], 1ULL << (en & 63));
+  };
+  if constexpr (kStash) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if ((uint32_t)lane + 64u * (uint32_t)k < nq) write_run((uint32_t)lane + 64u * (uint32_t)k, k);
+  } else {
+    for (uint32_t t = lane; t < nq; t += 64) write_run(t, 0);
   }
   __syncthreads();
 
@@ -1056,7 +1117,7 @@ __device__ __forceinline__ void load_lines(const QF& qf, uint64_t r, uint64_t s,
 // else loaded here.  With `has_next`, the lines of the region (rn, sn, en)
 // are loaded into L as soon as this region's lines are in LDS, so they
 // arrive while this one is rebuilt.
-template <int W>
+template <int W, bool kStash = true>
 __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
                               Smem<W>& sm, uint32_t* lcnt, bool concurrent, uint64_t tag, Lines<W / 64>& L,
@@ -1174,7 +1235,7 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
   }
   if constexpr (W == kWinFast) {
     if (shift_ok && fast && kind == kAdd)
-      return rebuild_insert_shift<W>(qf, B0, R, nwin, src, m, sm, occ_w[0], used_w[0], tm_w[0], table_end_in_window);
+      return rebuild_insert_shift<W, kStash>(qf, B0, R, nwin, src, m, sm, occ_w[0], used_w[0], tm_w[0], table_end_in_window);
   }
   if (fast) {
     // old run and new length of every listed quotient, one quotient per lane
@@ -1361,12 +1422,12 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
 }
 
 // rebuild_region_lines with the lines loaded here
-template <int W>
+template <int W, bool kStash = true>
 __device__ __forceinline__ int rebuild_region(const QF& qf, uint64_t r, const uint64_t* __restrict__ items,
                                               const uint64_t* __restrict__ counts, uint64_t s, uint64_t e, int mode,
                                               Smem<W>& sm, uint32_t* lcnt, bool concurrent, uint64_t tag = 0) {
   Lines<W / 64> L;
-  return rebuild_region_lines<W>(qf, r, items, counts, s, e, mode, sm, lcnt, concurrent, tag, L, false, false, 0, 0, 0);
+  return rebuild_region_lines<W, kStash>(qf, r, items, counts, s, e, mode, sm, lcnt, concurrent, tag, L, false, false, 0, 0, 0);
 }
 
 }  // namespace
@@ -1758,7 +1819,7 @@ gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode)
     region_hash_bounds(g, qf, f, r, items, s, e, tag);
     int res = kOk;
     if (!ok) res = s < e ? kNeedWide : kOk;
-    else if (s < e) res = rebuild_region<kWinFast>(qf, r, items, nullptr, s, e, mode, sm, nullptr, true, tag);
+    else if (s < e) res = rebuild_region<kWinFast, false>(qf, r, items, nullptr, s, e, mode, sm, nullptr, true, tag);
     __syncthreads();
     if (lane == 0) {
       if (res == kNeedWide) {
