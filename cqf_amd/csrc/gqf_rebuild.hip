@@ -49,10 +49,17 @@ constexpr int32_t kNegInf = INT32_MIN / 4;
 enum RebuildMode : int { kInsert = 0, kDelete = 1, kSeq = 2 };
 enum RebuildResult : int { kOk = 0, kNeedWide = 1, kNoSpace = 2 };
 
+// Run lengths of the listed quotients: one byte in the fast window (a
+// longer run sends the region to the wider windows, which keep 16 bits)
+template <int W>
+using RunLen = std::conditional_t<W == kWinFast, uint8_t, uint16_t>;
+template <int W>
+constexpr uint32_t kRunLenMax = W == kWinFast ? 255u : 65535u;
+
 template <int W>
 struct Smem {
   uint32_t sin[W * 17];   // old slots: slot p at byte sin_off(p), rows of 68 bytes (bank-conflict free)
-  uint32_t sout[(W + 1) * 17];  // new slots, same rows, and a spare row (a shift past row W - 1 lands there)
+  uint32_t sout[W * 17];  // new slots, same rows (word 16 of a row, its pad, takes junk writes)
   uint64_t run_in[W];
   // arrays whose lifetimes do not overlap share space
   union {
@@ -66,7 +73,6 @@ struct Smem {
   };
   int32_t xold[W + 1];  // end of the runs of quotients < 64 j (window-relative, >= 64 j - 1)
   int32_t xnew[W + 1];
-  uint64_t used_out[W];  // shift path: new used-slot masks
   union {
     uint32_t items[kItemCap];  // fast path: hash - first hash of the region
     struct {
@@ -76,10 +82,10 @@ struct Smem {
   };
   // one entry per distinct quotient among the items (fast path); the
   // quotient itself is items[tit0[t]] >> 8
-  uint16_t tit0[kItemCap];   // first item of the quotient
+  uint8_t tit0[kItemCap];    // first item of the quotient (< kItemCap = 256)
   int16_t tst[kItemCap];     // old run start
-  uint16_t tol[kItemCap];    // old run length (0: quotient was empty)
-  uint16_t tnl[kItemCap];    // new run length
+  RunLen<W> tol[kItemCap];   // old run length (0: quotient was empty)
+  RunLen<W> tnl[kItemCap];   // new run length
   uint32_t nq;
   int32_t first_invalid;
   int32_t xend;
@@ -183,7 +189,7 @@ __device__ __forceinline__ void lean_write(Smem<W>& sm, uint32_t lo, uint32_t hi
   }
   const uint32_t bn = sin_off(st_n);
   const uint32_t rn = 64u - (uint32_t)(st_n & 63);
-  const uint32_t spare = (uint32_t)(W * 68) + (uint32_t)lane_id();
+  const uint32_t spare = (uint32_t)lane_id() * 68u + 64u;  // the pad byte of row `lane`, never stored
   auto put_at = [&](uint32_t at, bool live, uint32_t v) {
     const uint32_t a = bn + at + (at >= rn ? 4u : 0u);
     out[live ? a : spare] = (uint8_t)v;
@@ -681,9 +687,13 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       atomicMin(&sm.first_invalid, -1);
       st = 0;
     }
+    if (ol > kRunLenMax<W> || nl > kRunLenMax<W>) {  // redone in a wider window
+      atomicMin(&sm.first_invalid, -1);
+      ol = nl = 0;
+    }
     sm.tst[t] = (int16_t)st;
-    sm.tol[t] = (uint16_t)ol;
-    sm.tnl[t] = (uint16_t)nl;
+    sm.tol[t] = (RunLen<W>)ol;
+    sm.tnl[t] = (RunLen<W>)nl;
   };
   sm.xnew[j] = 0;
   __syncthreads();
@@ -782,14 +792,14 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
 
   // ---- clear the rows and bit words to rewrite
   for (uint32_t c = lane; c < (uint32_t)bstar * 17; c += 64) sm.sout[c] = 0;
-  if (j < bstar) {
-    sm.run_out[j] = 0;
-    sm.used_out[j] = 0;
-  }
+  if (j < bstar) sm.run_out[j] = 0;
   __syncthreads();
 
   // ---- lane j moves the untouched bytes of slot-block j and records the
-  // new start of each of its events' runs
+  // new start of each of its events' runs; used_new is the block's new
+  // used-slot mask (the new runs' slots and what block j - 1 pushes in
+  // included), which only lane j writes and reads
+  uint64_t used_new = 0;
   if (j < bstar) {
     const uint64_t run_j = sm.run_in[j];
     const int32_t base = 64 * j;
@@ -888,9 +898,7 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
         rlo |= (uint64_t)rw[k] << (8 * k);
         if (8 * k + 31 >= 64) rhi |= (uint64_t)rw[k] >> (64 - 8 * k);
       }
-      // the block's whole used mask (the new runs' slots and what block j - 1
-      // pushes in included): lane j is its only writer
-      sm.used_out[j] = (used | ((uint64_t)nz[1] << 32 | nz[0])) & (used | gqf_mask((uint64_t)vlim));
+      used_new = (used | ((uint64_t)nz[1] << 32 | nz[0])) & (used | gqf_mask((uint64_t)vlim));
       atomicOr((unsigned long long*)&sm.run_out[j], rlo);
       if (e) {
         if (j + 1 < bstar) {  // always, by the choice of b*
@@ -954,7 +962,7 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
         if (ee > 0) mz |= 1ULL << i;
         if ((zero_ok >> i) & 1ULL) ee = ee > 0 ? ee - 1 : 0;
       }
-      sm.used_out[j] = (used | mz) & (used | gqf_mask((uint64_t)vlim));
+      used_new = (used | mz) & (used | gqf_mask((uint64_t)vlim));
     }
   }
   __syncthreads();
@@ -1035,7 +1043,7 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   if (j < bstar) {
     GQF_GLOBAL uint4* g = gqf_gptr_mut((uint4*)(qf.blocks + B0 * GQF_BLOCK_BYTES));
     const uint64_t run = sm.run_out[j];
-    const uint64_t un = sm.used_out[j];
+    const uint64_t un = used_new;
     const uint32_t* row = sm.sout + 17 * j;
     GQF_GLOBAL uint4* b = g + (size_t)j * (GQF_BLOCK_BYTES / 16);
     b[0] = make_uint4((uint32_t)occ_new, (uint32_t)(occ_new >> 32), (uint32_t)run, (uint32_t)(run >> 32));
@@ -1200,7 +1208,7 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
       const uint64_t hm = __ballot(head);
       if (head) {
         const uint32_t t = carry + (uint32_t)__popcll(hm & lt);
-        sm.tit0[t] = (uint16_t)i;
+        sm.tit0[t] = (uint8_t)i;
         atomicOr((unsigned long long*)&sm.tmask[h >> 14], 1ULL << ((h >> 8) & 63));
       }
       carry += (uint32_t)__popcll(hm);
@@ -1253,9 +1261,13 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
         st_o = q;
         old_len = 0;
       }
+      if (old_len > kRunLenMax<W> || nl > kRunLenMax<W>) {  // redone in a wider window
+        atomicMin(&sm.first_invalid, -1);
+        old_len = nl = 0;
+      }
       sm.tst[t] = (int16_t)st_o;
-      sm.tol[t] = (uint16_t)old_len;
-      sm.tnl[t] = (uint16_t)nl;
+      sm.tol[t] = (RunLen<W>)old_len;
+      sm.tnl[t] = (RunLen<W>)nl;
     }
   }
   __syncthreads();
