@@ -346,9 +346,27 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
 // (WRITE = false) returns the block's max-plus aggregate and new occupieds
 // word; pass 2 (WRITE = true) writes the runs into raw from xnew[j].
 
+// Start and end bits of the runs a pass-2 walk writes from block j's
+// quotients, inside block j (a run that leaves the block ends at bit 63).
+// The runs are disjoint and ascend, so (e - s) | e sets exactly their
+// slots; with the slots that earlier blocks' runs cover (the new offset)
+// that is block j's new used-slot mask, without a walk over its slots.
+struct RunBits {
+  uint64_t s = 0, e = 0;
+  __device__ inline void add(int32_t rs, int32_t re) {  // run [rs, re] relative to the block, rs >= 0
+    if (rs < 64) {
+      s |= 1ULL << rs;
+      e |= 1ULL << (re < 63 ? re : 63);
+    }
+  }
+  __device__ inline uint64_t used(uint32_t off) const {  // off: leading slots covered by earlier runs
+    return off >= 64 ? ~0ULL : (gqf_mask(off) | (e - s) | e);
+  }
+};
+
 template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const Items& src, uint32_t it, uint32_t ie,
-                           uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc) {
+                           uint64_t occ, int32_t& agg_a, int32_t& agg_b, uint64_t& newocc, RunBits& rb) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
   const int32_t blk_base = 64 * j;
@@ -417,6 +435,7 @@ __device__ __forceinline__ bool walk_block(Smem<W>& sm, int j, int nwin, const I
       if (WRITE) {
         const int32_t en_n = st_n + (int32_t)len - 1;
         atomicOr((unsigned long long*)&sm.run_out[en_n >> 6], 1ULL << (en_n & 63));
+        rb.add(st_n - blk_base, en_n - blk_base);
         xn = en_n;
       }
     }
@@ -471,7 +490,7 @@ __device__ __forceinline__ bool locate_old_run(const Smem<W>& sm, int nwin, int3
 template <int W, bool WRITE>
 __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, uint32_t tbase, uint64_t occ,
                                                  uint64_t touched, int32_t& agg_a, int32_t& agg_b,
-                                                 uint64_t& newocc) {
+                                                 uint64_t& newocc, RunBits& rb) {
   const uint8_t* in = (const uint8_t*)sm.sin;
   uint8_t* out = (uint8_t*)sm.sout;
   const int32_t blk_base = 64 * j;
@@ -537,6 +556,7 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
       if (WRITE) {
         const int32_t en_n = st_n + (int32_t)len - 1;
         atomicOr((unsigned long long*)&sm.run_out[en_n >> 6], 1ULL << (en_n & 63));
+        rb.add(st_n - blk_base, en_n - blk_base);
         xn = en_n;
       }
     }
@@ -1288,8 +1308,9 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
           it = fast ? tbase : sm.blk.start[j];
           ie = fast ? 0u : it + sm.blk.cnt[j];
         }
-        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occ_new[k])
-                             : walk_block<W, false>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occ_new[k]);
+        RunBits unused;
+        const bool ok = fast ? walk_block_table<W, false>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occ_new[k], unused)
+                             : walk_block<W, false>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occ_new[k], unused);
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
       int32_t ia = a, ib = b;
@@ -1372,6 +1393,7 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
   }
   __syncthreads();
   const int nrelay = bstar < nwin ? bstar : nwin;
+  RunBits rbits[kRounds];
 #pragma unroll
   for (int k = 0; k < kRounds; k++) {
     const int j = lane + 64 * k;
@@ -1383,8 +1405,8 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
       }
       int32_t a, b;
       uint64_t occn;
-      if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occn);
-      else walk_block<W, true>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occn);
+      if (fast) walk_block_table<W, true>(sm, j, nwin, it, occ_w[k], tm_w[k], a, b, occn, rbits[k]);
+      else walk_block<W, true>(sm, j, nwin, src, it, ie, occ_w[k], a, b, occn, rbits[k]);
     }
   }
   __syncthreads();
@@ -1417,8 +1439,10 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
           occ = occ_new[k];
         }
         const uint64_t run = sm.run_out[j];
+        // the meeting block's slots are laid out as before (everything up to
+        // its unchanged end-before is covered, the rest is copied)
         const uint64_t lim = qf.xnslots - (B0 + (uint64_t)j) * 64;
-        const uint64_t used = gqf_used_mask(occ, run, off, lim < 64 ? (uint32_t)lim : 64u);
+        const uint64_t used = j == bstar ? used_w[k] : rbits[k].used(off) & (lim < 64 ? gqf_mask(lim) : ~0ULL);
         const uint32_t* row = sm.sout + 17 * j;
         GQF_GLOBAL uint4* b = g + (size_t)j * (GQF_BLOCK_BYTES / 16);
         b[0] = make_uint4((uint32_t)occ, (uint32_t)(occ >> 32), (uint32_t)run, (uint32_t)(run >> 32));
