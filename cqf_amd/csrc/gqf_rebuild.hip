@@ -144,67 +144,76 @@ struct Items {
   }
 };
 
-// The straight-line insert merge (one or two items with distinct hashes
-// on a run of at most 8 count-1 counters, see merge_quotient), split in
-// two so a caller can keep what the reading half finds in registers until
-// the writing half: lo / hi = the old run's bytes, meta = 1 << 31 |
-// old_len << 20 | n << 18 | r1 << 8 | r0 (r1 = 0x200 when n = 1).  Bytes
-// past the run read as sentinels 0x100 + k (ascending, above every byte);
-// writes of absent bytes go to the lane's byte of the spare row.
+// The straight-line insert merge (one or two items with distinct hashes on
+// a run of count-1 counters, at most 8 slots once merged; see
+// merge_quotient), split in two so a caller can keep what the reading half
+// finds in registers until the writing half.  lean_read reads the old run
+// as aligned words with the row's pad word dropped, checks it, and returns
+// the merged run little-endian in lo / hi (byte k: slot k of the new run)
+// with its length; lean_write ORs those bytes into the (cleared) new rows,
+// at most three words.  A row's 64 slots end on a word boundary, so the pad
+// word of the row holding slot st is word 16 - (st & 63) / 4 counted from
+// the word holding st.
 template <int W>
 __device__ __forceinline__ bool lean_read(const Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
                                           int32_t st_o, uint32_t old_len, uint32_t h_first, uint32_t& lo,
-                                          uint32_t& hi, uint32_t& meta) {
-  const uint8_t* in = (const uint8_t*)sm.sin;
+                                          uint32_t& hi, uint32_t& len) {
   const uint32_t n = it1 - it0;
+  len = old_len + n;
   const uint32_t r0 = h_first & 0xFFu;
   const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : 0x200u;
-  const uint32_t bo = sin_off(st_o);
-  const uint32_t ro = 64u - (uint32_t)(st_o & 63);  // bytes of the run's first row
+  // the 8 slots from st_o (past the run: whatever follows, masked below)
+  const uint32_t a = sin_off(st_o);
+  const uint32_t* wp = sm.sin + (a >> 2);
+  const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+  const uint32_t pad = 16u - ((uint32_t)(st_o & 63) >> 2);
+  const uint32_t v1 = pad == 1 ? w2 : w1;
+  const uint32_t v2 = pad <= 2 ? w3 : w2;
+  const uint32_t sh = (a & 3u) * 8u;
+  const uint32_t x0 = __builtin_amdgcn_alignbit(v1, w0, sh);
+  const uint32_t x1 = __builtin_amdgcn_alignbit(v2, v1, sh);
+  // bytes past the run as sentinels 0x100 + k (ascending, above every byte)
   uint32_t ob[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint32_t b = in[bo + (uint32_t)k + ((uint32_t)k >= ro ? 4u : 0u)];
+    const uint32_t b = ((k < 4 ? x0 : x1) >> (8 * (k & 3))) & 0xFFu;
     ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
   }
-  bool simple = n == 1 || r1 > r0;
+  bool simple = len <= 8 && (n == 1 || r1 > r0);
 #pragma unroll
   for (int k = 1; k < 8; k++) simple = simple && ob[k] > ob[k - 1];
-  lo = (ob[0] & 0xFFu) | (ob[1] & 0xFFu) << 8 | (ob[2] & 0xFFu) << 16 | (ob[3] & 0xFFu) << 24;
-  hi = (ob[4] & 0xFFu) | (ob[5] & 0xFFu) << 8 | (ob[6] & 0xFFu) << 16 | (ob[7] & 0xFFu) << 24;
-  meta = 1u << 31 | old_len << 20 | n << 18 | r1 << 8 | r0;
-  return simple;
-}
-
-template <int W>
-__device__ __forceinline__ void lean_write(Smem<W>& sm, uint32_t lo, uint32_t hi, uint32_t meta, int32_t st_n) {
-  uint8_t* out = (uint8_t*)sm.sout;
-  const uint32_t old_len = (meta >> 20) & 0xFu, n = (meta >> 18) & 3u;
-  const uint32_t r0 = meta & 0xFFu, r1 = (meta >> 8) & 0x3FFu;
-  uint32_t ob[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint32_t b = ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
-    ob[k] = (uint32_t)k < old_len ? b : 0x100u + (uint32_t)k;
-  }
-  const uint32_t bn = sin_off(st_n);
-  const uint32_t rn = 64u - (uint32_t)(st_n & 63);
-  const uint32_t spare = (uint32_t)lane_id() * 68u + 64u;  // the pad byte of row `lane`, never stored
-  auto put_at = [&](uint32_t at, bool live, uint32_t v) {
-    const uint32_t a = bn + at + (at >= rn ? 4u : 0u);
-    out[live ? a : spare] = (uint8_t)v;
-  };
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-    put_at((uint32_t)k + (r0 < ob[k] ? 1u : 0u) + (r1 < ob[k] ? 1u : 0u), (uint32_t)k < old_len, ob[k]);
+  // r0 goes after the old bytes <= r0, r1 after those <= r1 and r0
   uint32_t a0 = 0, a1 = 1;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     a0 += ob[k] <= r0 ? 1u : 0u;
     a1 += ob[k] <= r1 ? 1u : 0u;
   }
-  put_at(a0, true, r0);
-  put_at(a1, n == 2, r1);
+  auto insert = [](uint64_t v, uint32_t at, uint32_t r) {  // byte r at `at`, the bytes from `at` up one
+    at = at < 7u ? at : 7u;                                  // (only a run that is not simple passes 7)
+    const uint64_t m = gqf_mask(8u * at);
+    return (v & m) | ((uint64_t)r << (8u * at)) | ((v & ~m) << 8);
+  };
+  uint64_t x = ((uint64_t)x1 << 32 | x0) & gqf_mask(8u * old_len);
+  x = insert(x, a0, r0);
+  if (n == 2) x = insert(x, a1, r1);
+  lo = (uint32_t)x;
+  hi = (uint32_t)(x >> 32);
+  return simple;
+}
+
+template <int W>
+__device__ __forceinline__ void lean_write(Smem<W>& sm, uint32_t lo, uint32_t hi, uint32_t len, int32_t st_n) {
+  const uint32_t a = sin_off(st_n);
+  const uint32_t sh = (a & 3u) * 8u;
+  const uint64_t u = (uint64_t)lo << sh, t = (uint64_t)hi << sh;
+  const uint32_t y0 = (uint32_t)u, y1 = (uint32_t)(u >> 32) | (uint32_t)t, y2 = (uint32_t)(t >> 32);
+  const uint32_t pad = 16u - ((uint32_t)(st_n & 63) >> 2);
+  uint32_t* wp = sm.sout + (a >> 2);
+  const uint32_t used = (a & 3u) + len;  // bytes of the three words the run takes
+  atomicOr(&wp[0], y0);
+  if (used > 4) atomicOr(&wp[pad == 1 ? 2 : 1], y1);
+  if (used > 8) atomicOr(&wp[pad <= 2 ? 3 : 2], y2);
 }
 
 // ---------------------------------------------------------------------------
@@ -231,14 +240,13 @@ __device__ __forceinline__ uint32_t merge_quotient(Smem<W>& sm, const Items& src
   // most 8 count-1 counters (strictly ascending bytes, see below) -- nearly
   // every quotient of a batch.  The new run is the old bytes and the new
   // ones merged, an old byte before an equal new one (r r is the count-2
-  // counter of r).  Straight-line code, the same on every lane: bytes past
-  // the run read as sentinels 0x100 + k (ascending, above every byte), and
-  // writes of absent bytes go to the lane's byte of the spare row.
+  // counter of r).  Straight-line code, the same on every lane (lean_read,
+  // lean_write), for merged runs of at most 8 slots.
   if (src.kind == kAdd && it1 - it0 <= 2 && old_len <= 8) {
-    uint32_t lo, hi, meta;
-    if (lean_read(sm, src, it0, it1, st_o, old_len, h_first, lo, hi, meta)) {
-      if (WRITE) lean_write(sm, lo, hi, meta, st_n);
-      return old_len + (it1 - it0);
+    uint32_t lo, hi, len;
+    if (lean_read(sm, src, it0, it1, st_o, old_len, h_first, lo, hi, len)) {
+      if (WRITE) lean_write(sm, lo, hi, len, st_n);
+      return len;
     }
   }
   // Common case: one item adding or removing 1 on a run (at most 8 slots) of
@@ -685,9 +693,10 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   // ---- event position, old length and new length of every listed quotient.
   // With kStash, lane takes quotients lane + 64 k in round k (at most
   // kItemCap = 4 x 64 of them), and a quotient that takes the straight-line
-  // merge keeps its old bytes and new remainders in registers for the
-  // writing pass (lean_* words; meta 0: the general merge).  Without it (a
-  // caller short of registers) every quotient takes the general merge.
+  // merge keeps its merged bytes in registers for the writing pass
+  // (lean_lo / lean_hi; lean_meta 0: merge_quotient).  Without it (a caller
+  // short of registers) both passes call merge_quotient, which redoes the
+  // straight-line merge where it applies.
   static_assert(kItemCap <= 4 * 64, "four rounds of listed quotients");
   uint32_t lean_lo[4], lean_hi[4], lean_meta[4] = {0, 0, 0, 0};
   auto measure = [&](uint32_t t, int k) {
@@ -698,8 +707,9 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
     uint32_t ol = 0, nl = 0;
     if (locate_run_or_slot(sm, nwin, q, st, ol)) {
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      if (kStash && it1 - it0 <= 2 && ol <= 8 && lean_read(sm, src, it0, it1, st, ol, h0, lean_lo[k], lean_hi[k], lean_meta[k]))
-        nl = ol + (it1 - it0);
+      uint32_t ln;
+      if (kStash && it1 - it0 <= 2 && ol <= 8 && lean_read(sm, src, it0, it1, st, ol, h0, lean_lo[k], lean_hi[k], ln))
+        nl = ln, lean_meta[k] = 1;
       else
         nl = merge_quotient<W, false>(sm, src, it0, it1, st, ol, 0, h0), lean_meta[k] = 0;
       atomicAdd((uint32_t*)&sm.xnew[st >> 6], 1u);  // events per slot-block (st < 64 nwin)
@@ -997,7 +1007,7 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       return;
     }
     if (kStash && lean_meta[k]) {
-      lean_write(sm, lean_lo[k], lean_hi[k], lean_meta[k], sn);
+      lean_write(sm, lean_lo[k], lean_hi[k], nl, sn);
     } else {
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
       merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sn, src.hash(sm.items, sm.tit0[t]));
