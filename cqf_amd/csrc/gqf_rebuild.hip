@@ -109,6 +109,42 @@ __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   b = b + b2;
 }
 
+// Inclusive scans over the 64 lanes of the wave on DPP row shifts (1, 2,
+// 4, 8 within each row of 16 lanes) and row broadcasts (lane 15 into rows 1
+// and 3, then lane 31 into rows 2 and 3): six VALU steps that read the
+// neighbouring lanes' registers, no LDS round trip.  All lanes must be
+// active.  Lanes a shift reaches past their row's start read the identity
+// (0 for sums; kNegInf, 0 for compose maps).
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}
+template <int kCtrl, int kRows>
+__device__ __forceinline__ void scan_step_compose(int32_t& a, int32_t& b) {
+  const int32_t pa = __builtin_amdgcn_update_dpp(kNegInf, a, kCtrl, kRows, 0xf, false);
+  const int32_t pb = __builtin_amdgcn_update_dpp(0, b, kCtrl, kRows, 0xf, false);
+  const int32_t na = pa + b;  // compose(pa, pb, a, b)
+  a = na > a ? na : a;
+  b = pb + b;
+}
+__device__ __forceinline__ void wave_scan_compose(int32_t& a, int32_t& b) {
+  scan_step_compose<0x111, 0xf>(a, b);
+  scan_step_compose<0x112, 0xf>(a, b);
+  scan_step_compose<0x114, 0xf>(a, b);
+  scan_step_compose<0x118, 0xf>(a, b);
+  scan_step_compose<0x142, 0xa>(a, b);
+  scan_step_compose<0x143, 0xc>(a, b);
+}
+// the value of the lane below (lane 0: fill); wave_shr:1
+__device__ __forceinline__ int32_t wave_shr1(int32_t x, int32_t fill) {
+  return __builtin_amdgcn_update_dpp(fill, x, 0x138, 0xf, 0xf, false);
+}
+
 
 // The region's items: cached relative hashes in LDS, or (when the region
 // holds more than kItemCap) read from global memory.  Each item changes its
@@ -605,18 +641,29 @@ __device__ __forceinline__ bool locate_run_or_slot(const Smem<W>& sm, int nwin, 
   const int need = occupied ? before + 1 : before;            // run ends to find after xold[j]
   int32_t prev = sm.xold[j], end = -1;
   if (need > 0) {
+    // the need-th run end after xold[j]; for an occupied q that is q's own
+    // run end, and the one before it (the highest set bit below, or the
+    // last one of an earlier word) ends q's predecessor
     const int32_t pos = sm.xold[j] + 1;
     int bj = pos >> 6;
     if (bj >= nwin) return false;
     uint64_t w = sm.run_in[bj] & (~0ULL << (pos & 63));
     int seen = 0;
+    int32_t last = prev;
     for (;;) {
       const int c = __popcll(w);
-      if (before >= 1 && before - 1 >= seen && before - 1 < seen + c) prev = 64 * bj + gqf_select64(w, before - 1 - seen);
       if (need - 1 < seen + c) {
-        if (occupied) end = 64 * bj + gqf_select64(w, need - 1 - seen);
+        const int e = gqf_select64(w, need - 1 - seen);
+        if (occupied) {
+          end = 64 * bj + e;
+          const uint64_t below = w & gqf_mask((uint64_t)e);
+          if (before >= 1) prev = below ? 64 * bj + 63 - __builtin_clzll(below) : last;
+        } else {
+          prev = 64 * bj + e;
+        }
         break;
       }
+      if (c) last = 64 * bj + 63 - __builtin_clzll(w);
       seen += c;
       if (++bj >= nwin) return false;
       w = sm.run_in[bj];
@@ -743,12 +790,7 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   uint32_t e0, e1;
   {
     const uint32_t c = (uint32_t)sm.xnew[j];
-    uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t u = (uint32_t)__shfl_up((int)incl, d, 64);
-      if (lane >= d) incl += u;
-    }
+    const uint32_t incl = wave_scan_add(c);
     e0 = incl - c;
     e1 = incl;
   }
@@ -776,20 +818,10 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
   // inclusive scan of the maps over the lanes; the shift entering block j
   // is the exclusive prefix applied to 0
   int32_t ia = A, ib = Bm;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int32_t pa = __shfl_up(ia, d, 64);
-    const int32_t pb = __shfl_up(ib, d, 64);
-    if (lane >= d) {
-      const int32_t na = pa + ib;
-      ia = na > ia ? na : ia;
-      ib = pb + ib;
-    }
-  }
-  int32_t ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64);
-  if (lane == 0) { ea = kNegInf; eb = 0; }
+  wave_scan_compose(ia, ib);
+  const int32_t ea = wave_shr1(ia, kNegInf), eb = wave_shr1(ib, 0);
   const int32_t delta_in = ea > eb ? ea : eb;  // max(a, 0 + b)
-  const int32_t la = __shfl(ia, 63, 64), lb = __shfl(ib, 63, 64);
+  const int32_t la = __builtin_amdgcn_readlane(ia, 63), lb = __builtin_amdgcn_readlane(ib, 63);
   const int32_t delta_end = la > lb ? la : lb;
 
   // ---- b*: the first block at or after R, past the last event's old run,
@@ -1037,14 +1069,8 @@ __device__ __forceinline__ int rebuild_insert_shift(const QF& qf, uint64_t B0, i
       c_pre = (uint32_t)__popcll(sm.run_in[j] & gqf_mask((uint64_t)(upto < 64 ? upto : 64)));
     }
   }
-  uint32_t inc_occ = c_occ, inc_re = c_re, tot_pre = c_pre;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t a1 = (uint32_t)__shfl_up((int)inc_occ, d, 64);
-    const uint32_t a2 = (uint32_t)__shfl_up((int)inc_re, d, 64);
-    if (lane >= d) { inc_occ += a1; inc_re += a2; }
-    tot_pre += (uint32_t)__shfl_xor((int)tot_pre, d, 64);
-  }
+  const uint32_t inc_occ = wave_scan_add(c_occ), inc_re = wave_scan_add(c_re);
+  const uint32_t tot_pre = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_add(c_pre), 63);
   sm.xnew[j] = (int32_t)(inc_re - c_re);  // run ends in blocks < j
   if (lane == 63) sm.xnew[64] = (int32_t)inc_re;
   __syncthreads();
@@ -1254,13 +1280,7 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
   uint32_t tbase;
   {
     const uint32_t c = fast ? (lane < nreg ? (uint32_t)__popcll(sm.tmask[lane]) : 0u) : sm.blk.cnt[lane];
-    uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
-    tbase = incl - c;
+    tbase = wave_scan_add(c) - c;
     if (!fast) sm.blk.start[lane] = tbase;
   }
   // the touched quotients of this lane's blocks, in registers from here on
@@ -1324,25 +1344,15 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
         if (!ok) atomicMin(&sm.first_invalid, j);
       }
       int32_t ia = a, ib = b;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int32_t pa = __shfl_up(ia, d, 64);
-        const int32_t pb = __shfl_up(ib, d, 64);
-        if (lane >= d) {
-          const int32_t na = pa + ib;
-          ia = na > ia ? na : ia;
-          ib = pb + ib;
-        }
-      }
-      int32_t ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64);
-      if (lane == 0) { ea = kNegInf; eb = 0; }
+      wave_scan_compose(ia, ib);
+      const int32_t ea = wave_shr1(ia, kNegInf), eb = wave_shr1(ib, 0);
       int32_t pa = ca, pb = cb;
       compose(pa, pb, ea, eb);
       if (j < nwin) {
         const int32_t x = pa > e_in + pb ? pa : e_in + pb;
         sm.xnew[j] = x > 64 * j - 1 ? x : 64 * j - 1;
       }
-      compose(ca, cb, __shfl(ia, 63, 64), __shfl(ib, 63, 64));
+      compose(ca, cb, __builtin_amdgcn_readlane(ia, 63), __builtin_amdgcn_readlane(ib, 63));
     }
     if (lane == 0) {
       const int32_t x = ca > e_in + cb ? ca : e_in + cb;
