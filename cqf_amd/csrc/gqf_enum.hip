@@ -107,12 +107,7 @@ __device__ inline int64_t walk_items(const QF& f, uint64_t b, uint64_t q_lo, uin
 __device__ inline uint32_t wg_exclusive_scan(uint32_t v, uint32_t* total) {
   __shared__ uint32_t wsum[kThreads / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += t;
-  }
+  uint32_t incl = wave_scan_add(v);
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
   uint32_t before = 0, tot = 0;

@@ -35,3 +35,18 @@ static inline uint32_t gqf_batch_pieces(uint64_t n, uint64_t max_piece, uint64_t
 #define GQF_STATUS_CLUSTER_OVERFLOW QF_STATUS_CLUSTER_OVERFLOW  // push longer than the widest window
 #define GQF_STATUS_BAD_HASH QF_STATUS_BAD_HASH                  // pre-hashed key >= range; key ignored
 #define GQF_STATUS_POINT_FULL QF_STATUS_POINT_FULL              // a point insert found no empty slot
+
+// Inclusive sum over the 64 lanes of a wave on DPP row shifts (1, 2, 4, 8
+// within each row of 16 lanes) and row broadcasts (lane 15 into rows 1 and
+// 3, then lane 31 into rows 2 and 3): six VALU steps that read the
+// neighbouring lanes' registers, no LDS round trip.  All lanes must be
+// active.
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}

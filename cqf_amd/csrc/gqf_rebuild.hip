@@ -109,21 +109,9 @@ __device__ inline void compose(int32_t& a, int32_t& b, int32_t a2, int32_t b2) {
   b = b + b2;
 }
 
-// Inclusive scans over the 64 lanes of the wave on DPP row shifts (1, 2,
-// 4, 8 within each row of 16 lanes) and row broadcasts (lane 15 into rows 1
-// and 3, then lane 31 into rows 2 and 3): six VALU steps that read the
-// neighbouring lanes' registers, no LDS round trip.  All lanes must be
-// active.  Lanes a shift reaches past their row's start read the identity
-// (0 for sums; kNegInf, 0 for compose maps).
-__device__ __forceinline__ uint32_t wave_scan_add(uint32_t x) {
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return x;
-}
+// Inclusive compose scan over the 64 lanes, on the same DPP steps as
+// wave_scan_add (gqf_internal.h); lanes a shift reaches past their row's
+// start read the identity map (kNegInf, 0).  All lanes must be active.
 template <int kCtrl, int kRows>
 __device__ __forceinline__ void scan_step_compose(int32_t& a, int32_t& b) {
   const int32_t pa = __builtin_amdgcn_update_dpp(kNegInf, a, kCtrl, kRows, 0xf, false);

@@ -90,12 +90,7 @@ gqf_sort_scan_rows(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __res
   for (uint32_t base = 0; base < ntiles; base += kThreads) {
     uint32_t i = base + threadIdx.x;
     uint32_t v = i < ntiles ? row[i] : 0;
-    uint32_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
+    uint32_t incl = wave_scan_add(v);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t before = 0, total = 0;
@@ -124,12 +119,7 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* s, int count) {
     v[k] = d < count ? s[d] : 0;
     sum += v[k];
   }
-  uint32_t incl = sum;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t x = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += x;
-  }
+  uint32_t incl = wave_scan_add(sum);
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
   uint32_t run = incl - sum;
@@ -355,12 +345,7 @@ gqf_msd_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uin
     __syncthreads();
     // exclusive scan of the digit counts, one digit per thread
     const uint32_t c = t < radix ? cnt[t] : 0u;
-    uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t x = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += x;
-    }
+    uint32_t incl = wave_scan_add(c);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t before = 0;
@@ -423,12 +408,7 @@ __device__ __forceinline__ void bucket_pass(const V* src, V* dst, uint32_t m, in
       v[k] = tot[4 * lane + k];
       sum += v[k];
     }
-    uint32_t incl = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t x = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += x;
-    }
+    uint32_t incl = wave_scan_add(sum);
     uint32_t r = incl - sum;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -557,12 +537,7 @@ gqf_sort_buckets(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, ui
     const uint32_t c0 = d0 < nsub ? sub_end[d0] : 0u;
     const uint32_t c1 = d0 + 1 < nsub ? sub_end[d0 + 1] : 0u;
     const uint32_t sum = c0 + c1;
-    uint32_t incl = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t x = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += x;
-    }
+    uint32_t incl = wave_scan_add(sum);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t run = incl - sum;
@@ -717,12 +692,7 @@ gqf_rle_scan(uint32_t* __restrict__ tile_heads, uint32_t ntiles, uint64_t* __res
   for (uint32_t base = 0; base < ntiles; base += kThreads) {
     uint32_t i = base + threadIdx.x;
     uint32_t v = i < ntiles ? tile_heads[i] : 0;
-    uint32_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
+    uint32_t incl = wave_scan_add(v);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t before = 0, total = 0;
