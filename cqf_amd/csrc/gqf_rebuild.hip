@@ -1834,7 +1834,9 @@ __device__ inline void region_hash_bounds(const gqf_group_dev& g, const QF& qf, 
 // polled value.  (A workgroup is one wave, so its __syncthreads() is only a
 // wave barrier: it does not keep lanes in the same loop iteration, and a
 // per-lane loop condition would let the compiler split the lanes.)
-extern "C" __global__ void __launch_bounds__(64)
+// Three waves per SIMD (168 VGPRs, no spills) keep 12 workgroups per CU
+// resident, as many as the LDS allows.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 gqf_group_rebuild(gqf_group_dev g, const uint64_t* __restrict__ items, int mode) {
   __shared__ Smem<kWinFast> sm;
   const int lane = lane_id();
