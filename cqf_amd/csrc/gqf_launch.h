@@ -121,11 +121,13 @@ void gqf_launch_seq_adjust(const QF* qf_dev, const uint64_t* keys, uint64_t n, c
 // both even/odd phases (+ wide fallback passes) of a merge-rebuild over
 // the listed regions; n_items bounds the number of non-empty regions.
 // wide_flags: one byte per region; wide_count: GQF_WIDE_WORDS words (the
-// number of regions flagged in a phase, the first GQF_WIDE_LIST of them,
-// and a counter of finished wide workgroups); all zero between calls (the
-// kernels leave them so)
+// number of regions flagged in a phase, the first GQF_WIDE_LIST of them, a
+// counter of finished wide workgroups, a ticket counter and one finished
+// counter per class of the wide pass); all zero between calls (the kernels
+// leave them so)
 #define GQF_WIDE_LIST 64
-#define GQF_WIDE_WORDS (2 + GQF_WIDE_LIST)
+#define GQF_WIDE_MAX_CLASSES 8  // classes of the wide pass: ceil(ceil(256 / R) / 2), R >= 16
+#define GQF_WIDE_WORDS (3 + GQF_WIDE_LIST + GQF_WIDE_MAX_CLASSES)
 void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blocks, const uint64_t* items,
                         const uint64_t* counts, uint64_t n_items, const uint32_t* region_start,
                         const uint32_t* region_list, const uint32_t* list_count, int mode, uint8_t* wide_flags,

@@ -1541,10 +1541,9 @@ gqf_rebuild_phase_counts(const QF* __restrict__ qfp, const uint64_t* __restrict_
 // touch each other's blocks, so the flagged regions are cut into chains of
 // closer ones, and each chain runs on its own workgroup (up to
 // GQF_WIDE_LIST of them).  Up to GQF_WIDE_LIST flagged regions come from
-// the phase's list, ordered by rank; with more, workgroup 0 scans every
-// region's flag in order (683 dependent rounds at 2^28 slots).  Every
-// workgroup exits at once when nothing was flagged; otherwise the last one
-// to finish clears the count for the next phase.
+// the phase's list, ordered by rank; more go to gqf_rebuild_wide_class.
+// Every workgroup exits at once when nothing was flagged; otherwise the
+// last one to finish clears the count for the next phase.
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                  const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
@@ -1597,7 +1596,7 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
       }
     }
   } else {
-    return;  // more than GQF_WIDE_LIST flagged: the class launches take them (gqf_rebuild_wide_class)
+    return;  // more than GQF_WIDE_LIST flagged: the class launch takes them (gqf_rebuild_wide_class)
   }
   // the last workgroup to finish clears the count and the finish counter
   if (lane == 0) {
@@ -1609,31 +1608,58 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
   }
 }
 
-// The flagged regions of a phase when there are more than GQF_WIDE_LIST:
-// regions of one parity whose indices differ by at least `apart` (the
-// widest window, 256 blocks, in regions) have disjoint windows, so they
-// can be redone at once.  Region parity + 2 i is in class i mod ncls
-// (ncls = ceil(apart / 2)); the launcher runs the classes one after
-// another, each over the whole grid, and the last class's last workgroup
-// clears the count.  Nothing runs when GQF_WIDE_LIST or fewer were
-// flagged (gqf_rebuild_wide has taken them).
+// The flagged regions of a phase when there are more than GQF_WIDE_LIST
+// (loads near 95%): regions of one parity whose indices differ by at least
+// `apart` (the widest window, 256 blocks, in regions) have disjoint
+// windows, so region parity + 2 i is in class i mod ncls (ncls =
+// ceil(apart / 2)) and the classes run one after another, in ONE launch of
+// ncls x kWideClassGrid workgroups.  Each workgroup takes a ticket when it
+// starts; tickets c * kWideClassGrid .. (c + 1) * kWideClassGrid - 1 run
+// class c, after waiting for class c - 1's finished counter.  Every
+// workgroup of class c - 1 took its ticket before any of class c did, so it
+// is running or done, and it waits only on earlier classes: every wait
+// ends.  (A wait past kWideMaxPolls, which no correct run reaches, flags the
+// region as CLUSTER_OVERFLOW instead of rebuilding it.)  The last workgroup
+// to finish clears every counter for the next phase.  Nothing runs when
+// GQF_WIDE_LIST or fewer were flagged (gqf_rebuild_wide has taken them).
+constexpr uint32_t kWideMaxPolls = 1u << 24;
+
+__device__ inline bool wait_count(const uint32_t* ctr, uint32_t target, uint32_t max_polls) {
+  for (uint32_t k = 0; k < max_polls; k++) {
+    const uint32_t v = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)__builtin_amdgcn_readfirstlane(v) == target) return true;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  return false;
+}
+
 extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                        const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
                        int parity, int mode, uint8_t* __restrict__ wide_flags, uint32_t* __restrict__ wide_count,
-                       uint32_t cls, uint32_t ncls) {
+                       uint32_t ncls) {
   __shared__ Smem<kWinMid> smm;
   __shared__ Smem<kWinWide> sm;
   __shared__ uint32_t icnt[kItemCap];
+  __shared__ uint32_t ticket;
   const uint32_t nflag = *(volatile uint32_t*)wide_count;
   if (nflag <= GQF_WIDE_LIST) return;
   const QF qf = *qfp;
   const int lane = lane_id();
+  uint32_t* finished = &wide_count[1 + GQF_WIDE_LIST];
+  uint32_t* tickets = &wide_count[2 + GQF_WIDE_LIST];
+  uint32_t* class_done = &wide_count[3 + GQF_WIDE_LIST];
+  if (lane == 0) ticket = atomicAdd(tickets, 1u);
+  __syncthreads();
+  const uint32_t t = ticket;
+  const uint32_t cls = t / kWideClassGrid, w = t % kWideClassGrid;
+  const bool ok = cls == 0 || wait_count(&class_done[cls - 1], kWideClassGrid, kWideMaxPolls);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   const uint64_t nper = qf.nregions > (uint64_t)parity ? (qf.nregions - (uint64_t)parity + 1) / 2 : 0;
   // candidate j of this class is i = cls + ncls j, region parity + 2 i;
   // workgroups take 64 candidates at a time
   const uint64_t ncand = nper > cls ? (nper - cls + ncls - 1) / ncls : 0;
-  for (uint64_t j0 = (uint64_t)blockIdx.x * 64; j0 < ncand; j0 += (uint64_t)gridDim.x * 64) {
+  for (uint64_t j0 = (uint64_t)w * 64; j0 < ncand; j0 += (uint64_t)kWideClassGrid * 64) {
     const uint64_t j = j0 + (uint64_t)lane;
     const uint64_t r = (uint64_t)parity + 2 * ((uint64_t)cls + (uint64_t)ncls * j);
     uint64_t m = __ballot(j < ncand && wide_flags[r]);
@@ -1641,11 +1667,14 @@ gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
       const int l = __builtin_ctzll(m);
       m &= m - 1;
       const uint64_t rr = (uint64_t)parity + 2 * ((uint64_t)cls + (uint64_t)ncls * (j0 + (uint64_t)l));
-      const uint64_t s = region_start[rr], e = region_start[rr + 1];
-      int res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt, false);
-      __syncthreads();
-      if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt, false);
-      __syncthreads();
+      int res = kNeedWide;
+      if (ok) {
+        const uint64_t s = region_start[rr], e = region_start[rr + 1];
+        res = rebuild_region<kWinMid>(qf, rr, items, counts, s, e, mode, smm, icnt, false);
+        __syncthreads();
+        if (res == kNeedWide) res = rebuild_region<kWinWide>(qf, rr, items, counts, s, e, mode, sm, icnt, false);
+        __syncthreads();
+      }
       if (lane == 0) {
         wide_flags[rr] = 0;
         if (res == kNeedWide) atomicOr(qf.status, GQF_STATUS_CLUSTER_OVERFLOW);
@@ -1654,10 +1683,13 @@ gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
       __syncthreads();
     }
   }
-  if (cls + 1 == ncls && lane == 0) {
-    __threadfence();
-    if (atomicAdd(&wide_count[1 + GQF_WIDE_LIST], 1u) == gridDim.x - 1) {
-      wide_count[1 + GQF_WIDE_LIST] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) {
+    atomicAdd(&class_done[cls], 1u);
+    if (atomicAdd(finished, 1u) == gridDim.x - 1) {  // everyone is done: clear for the next phase
+      for (uint32_t c = 0; c < ncls; c++) class_done[c] = 0;
+      *tickets = 0;
+      *finished = 0;
       *(volatile uint32_t*)wide_count = 0;
     }
   }
@@ -1971,6 +2003,13 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blo
   const uint64_t per_parity = (nregions + 1) / 2;
   const uint32_t g = (uint32_t)(n_items < per_parity ? (n_items ? n_items : 1) : per_parity);
   const uint64_t stride = per_parity + 1;
+  // classes of the wide pass (regions whose widest windows are disjoint)
+  const uint32_t apart = (uint32_t)((kWinWide + region_blocks - 1) / region_blocks);
+  const uint32_t ncls = (apart + 1) / 2;
+  if (ncls < 1 || ncls > GQF_WIDE_MAX_CLASSES) {  // region_blocks is validated to 16..56 on the host
+    fprintf(stderr, "gqf: %u wide classes for %llu-block regions\n", ncls, (unsigned long long)region_blocks);
+    abort();
+  }
   mode |= gqf_rebuild_mode_flags();
   for (int parity = 0; parity < 2; parity++) {
     gqf_prof_begin("rebuild_phase", s);
@@ -1984,13 +2023,8 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blo
     gqf_prof_begin("rebuild_wide", s);
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(GQF_WIDE_LIST), dim3(64), 0, s, qf_dev, items, counts, region_start,
                        parity, mode, wide_flags, wide_count);
-    // more than GQF_WIDE_LIST flagged (loads near 95%): the classes of
-    // regions whose widest windows are disjoint, one launch each
-    const uint32_t apart = (uint32_t)((kWinWide + region_blocks - 1) / region_blocks);
-    const uint32_t ncls = (apart + 1) / 2;
-    for (uint32_t c = 0; c < ncls; c++)
-      hipLaunchKernelGGL(gqf_rebuild_wide_class, dim3(kWideClassGrid), dim3(64), 0, s, qf_dev, items, counts,
-                         region_start, parity, mode, wide_flags, wide_count, c, ncls);
+    hipLaunchKernelGGL(gqf_rebuild_wide_class, dim3(ncls * kWideClassGrid), dim3(64), 0, s, qf_dev, items, counts,
+                       region_start, parity, mode, wide_flags, wide_count, ncls);
     gqf_prof_end("rebuild_wide", s);
   }
 }

@@ -224,8 +224,8 @@ def test_wide_window_fallback_at_high_load(gpu, oracle, nbits):
     """56-block regions leave an 8-block push window, so at 95% load many
     regions outrun it and are redone by the wide pass: from its list of
     flagged regions at 2^16 slots (19 regions), and at 2^22 (1,171
-    regions) by the class launches, which redo the flagged regions whose
-    widest windows are disjoint at once.  Inserts and deletes must still
+    regions) by the class launch, whose classes redo the flagged regions
+    with disjoint widest windows at once, one class after another.  Inserts and deletes must still
     give the oracle's image."""
     from cqf_amd.gqf import to_device_u64, to_numpy_u64
     _say(f"case wide fallback q={nbits} R=56 load 0.95")
