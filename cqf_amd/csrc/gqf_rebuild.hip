@@ -2031,10 +2031,10 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blo
 
 // Workgroups per CU the persistent rebuild can keep resident (all of its
 // grid must be: see gqf_group_rebuild), and the grid for `per_cu` of them
-// (0: the default, clamped to what is resident).  The default is 8: LDS
-// would allow 12 (13.25 KB each of 160 KB), but 8 measured faster for the
-// 256-filter group insert (156 ms against 172 ms with 12); gqf_group_autotune
-// times the candidates for a group's own shape.
+// (0: the default, clamped to what is resident).  Up to 12 are resident
+// (3 waves per SIMD; 13,088 B of LDS each); the default is 8, and
+// gqf_group_autotune times the candidates for a group's own shape (10 for
+// the 256-filter k-mer workload).
 uint32_t gqf_group_rebuild_max_per_cu() {
   static int per_cu = 0;
   if (!per_cu) {
