@@ -178,6 +178,22 @@ struct Items {
 // at most three words.  A row's 64 slots end on a word boundary, so the pad
 // word of the row holding slot st is word 16 - (st & 63) / 4 counted from
 // the word holding st.
+// The 8 old slots from st (whatever follows a shorter run included), little
+// endian in x0 / x1: the words holding them read aligned, the row's pad
+// word dropped, and funnel-shifted to st's byte.
+template <int W>
+__device__ __forceinline__ void read_slots8(const Smem<W>& sm, int32_t st, uint32_t& x0, uint32_t& x1) {
+  const uint32_t a = sin_off(st);
+  const uint32_t* wp = sm.sin + (a >> 2);
+  const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
+  const uint32_t pad = 16u - ((uint32_t)(st & 63) >> 2);
+  const uint32_t v1 = pad == 1 ? w2 : w1;
+  const uint32_t v2 = pad <= 2 ? w3 : w2;
+  const uint32_t sh = (a & 3u) * 8u;
+  x0 = __builtin_amdgcn_alignbit(v1, w0, sh);
+  x1 = __builtin_amdgcn_alignbit(v2, v1, sh);
+}
+
 template <int W>
 __device__ __forceinline__ bool lean_read(const Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
                                           int32_t st_o, uint32_t old_len, uint32_t h_first, uint32_t& lo,
@@ -186,16 +202,8 @@ __device__ __forceinline__ bool lean_read(const Smem<W>& sm, const Items& src, u
   len = old_len + n;
   const uint32_t r0 = h_first & 0xFFu;
   const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : 0x200u;
-  // the 8 slots from st_o (past the run: whatever follows, masked below)
-  const uint32_t a = sin_off(st_o);
-  const uint32_t* wp = sm.sin + (a >> 2);
-  const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2], w3 = wp[3];
-  const uint32_t pad = 16u - ((uint32_t)(st_o & 63) >> 2);
-  const uint32_t v1 = pad == 1 ? w2 : w1;
-  const uint32_t v2 = pad <= 2 ? w3 : w2;
-  const uint32_t sh = (a & 3u) * 8u;
-  const uint32_t x0 = __builtin_amdgcn_alignbit(v1, w0, sh);
-  const uint32_t x1 = __builtin_amdgcn_alignbit(v2, v1, sh);
+  uint32_t x0, x1;
+  read_slots8(sm, st_o, x0, x1);
   // bytes past the run as sentinels 0x100 + k (ascending, above every byte)
   uint32_t ob[8];
 #pragma unroll
@@ -503,13 +511,19 @@ __device__ __forceinline__ bool locate_old_run(const Smem<W>& sm, int nwin, int3
   if (bj >= nwin) return false;
   uint64_t w = sm.run_in[bj] & (~0ULL << (pos & 63));
   int seen = 0;
+  int32_t last = prev;  // the last run end seen in earlier words
   for (;;) {
     const int c = __popcll(w);
-    if (rank >= 2 && rank - 2 >= seen && rank - 2 < seen + c) prev = 64 * bj + gqf_select64(w, rank - 2 - seen);
     if (rank - 1 < seen + c) {
-      end = 64 * bj + gqf_select64(w, rank - 1 - seen);
+      // q's run end; the one before it (the highest set bit below, or the
+      // last one of an earlier word) ends q's predecessor
+      const int e = gqf_select64(w, rank - 1 - seen);
+      end = 64 * bj + e;
+      const uint64_t below = w & gqf_mask((uint64_t)e);
+      if (rank >= 2) prev = below ? 64 * bj + 63 - __builtin_clzll(below) : last;
       break;
     }
+    if (c) last = 64 * bj + 63 - __builtin_clzll(w);
     seen += c;
     if (++bj >= nwin) return false;
     w = sm.run_in[bj];
@@ -567,16 +581,12 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
     } else {
       len = (uint32_t)(en_o - st_o + 1);
       if (WRITE) {
-        if (len <= 4) {
-          // all reads issued before the writes: one LDS round trip per run
-          const uint8_t b0 = in[sin_off(st_o)];
-          const uint8_t b1 = len > 1 ? in[sin_off(st_o + 1)] : 0;
-          const uint8_t b2 = len > 2 ? in[sin_off(st_o + 2)] : 0;
-          const uint8_t b3 = len > 3 ? in[sin_off(st_o + 3)] : 0;
-          out[sin_off(st_n)] = b0;
-          if (len > 1) out[sin_off(st_n + 1)] = b1;
-          if (len > 2) out[sin_off(st_n + 2)] = b2;
-          if (len > 3) out[sin_off(st_n + 3)] = b3;
+        if (len <= 8) {
+          // as words: the old slots read aligned, ORed into the cleared rows
+          uint32_t x0, x1;
+          read_slots8(sm, st_o, x0, x1);
+          const uint64_t x = ((uint64_t)x1 << 32 | x0) & gqf_mask(8u * len);
+          lean_write(sm, (uint32_t)x, (uint32_t)(x >> 32), len, st_n);
         } else {
           for (uint32_t k = 0; k < len; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
         }
