@@ -855,6 +855,8 @@ extern "C" int qf_reset_device(QF* qf, void* stream) {
 // Tuning knobs (0 keeps the current value): blocks per rebuild region and
 // radix digit bits of the bulk sort.  Takes effect for later calls on the
 // stream; synchronises.
+extern "C" uint32_t qf_set_wide_wait_bound(uint32_t max_polls) { return gqf_wide_wait_bound_set(max_polls); }
+
 extern "C" int qf_set_tuning(QF* qf, uint32_t region_blocks, uint32_t sort_digit_bits) {
   Filter* f = lookup(qf);
   if (!f) return -1;

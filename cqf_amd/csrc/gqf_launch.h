@@ -125,6 +125,8 @@ void gqf_launch_seq_adjust(const QF* qf_dev, const uint64_t* keys, uint64_t n, c
 // counter of finished wide workgroups, a ticket counter and one finished
 // counter per class of the wide pass); all zero between calls (the kernels
 // leave them so)
+// the class waits' poll bound (qf_set_wide_wait_bound); returns the previous one
+uint32_t gqf_wide_wait_bound_set(uint32_t max_polls);
 #define GQF_WIDE_LIST 64
 #define GQF_WIDE_MAX_CLASSES 8  // classes of the wide pass: ceil(ceil(256 / R) / 2), R >= 16
 #define GQF_WIDE_WORDS (3 + GQF_WIDE_LIST + GQF_WIDE_MAX_CLASSES)

@@ -31,6 +31,7 @@
 // window, in ascending region order, after the phase.
 #include <stdlib.h>
 
+#include <atomic>
 #include <type_traits>
 #include <utility>
 
@@ -1628,9 +1629,10 @@ gqf_rebuild_wide(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
 // class c, after waiting for class c - 1's finished counter.  Every
 // workgroup of class c - 1 took its ticket before any of class c did, so it
 // is running or done, and it waits only on earlier classes: every wait
-// ends.  (A wait past kWideMaxPolls, which no correct run reaches, flags the
-// region as CLUSTER_OVERFLOW instead of rebuilding it.)  The last workgroup
-// to finish clears every counter for the next phase.  Nothing runs when
+// ends.  (A wait past max_polls, 2^24 unless qf_set_wide_wait_bound says
+// otherwise, which no correct run reaches, flags the class's regions as
+// CLUSTER_OVERFLOW instead of rebuilding them.)  The last workgroup to
+// finish clears every counter for the next phase.  Nothing runs when
 // GQF_WIDE_LIST or fewer were flagged (gqf_rebuild_wide has taken them).
 constexpr uint32_t kWideMaxPolls = 1u << 24;
 
@@ -1647,7 +1649,7 @@ extern "C" __global__ void __launch_bounds__(64)
 gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ items,
                        const uint64_t* __restrict__ counts, const uint32_t* __restrict__ region_start,
                        int parity, int mode, uint8_t* __restrict__ wide_flags, uint32_t* __restrict__ wide_count,
-                       uint32_t ncls) {
+                       uint32_t ncls, uint32_t max_polls) {
   __shared__ Smem<kWinMid> smm;
   __shared__ Smem<kWinWide> sm;
   __shared__ uint32_t icnt[kItemCap];
@@ -1663,7 +1665,7 @@ gqf_rebuild_wide_class(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
   __syncthreads();
   const uint32_t t = ticket;
   const uint32_t cls = t / kWideClassGrid, w = t % kWideClassGrid;
-  const bool ok = cls == 0 || wait_count(&class_done[cls - 1], kWideClassGrid, kWideMaxPolls);
+  const bool ok = cls == 0 || wait_count(&class_done[cls - 1], kWideClassGrid, max_polls);
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   const uint64_t nper = qf.nregions > (uint64_t)parity ? (qf.nregions - (uint64_t)parity + 1) / 2 : 0;
   // candidate j of this class is i = cls + ncls j, region parity + 2 i;
@@ -2004,6 +2006,12 @@ void gqf_launch_region_lists(const QF* qf_dev, uint64_t nregions, uint64_t n, ui
   gqf_prof_end("region_bounds", s);
 }
 
+namespace {
+std::atomic<uint32_t> g_wide_max_polls{kWideMaxPolls};
+}  // namespace
+
+uint32_t gqf_wide_wait_bound_set(uint32_t max_polls) { return g_wide_max_polls.exchange(max_polls); }
+
 void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blocks, const uint64_t* items,
                         const uint64_t* counts, uint64_t n_items, const uint32_t* region_start,
                         const uint32_t* region_list, const uint32_t* list_count, int mode, uint8_t* wide_flags,
@@ -2034,7 +2042,8 @@ void gqf_launch_rebuild(const QF* qf_dev, uint64_t nregions, uint64_t region_blo
     hipLaunchKernelGGL(gqf_rebuild_wide, dim3(GQF_WIDE_LIST), dim3(64), 0, s, qf_dev, items, counts, region_start,
                        parity, mode, wide_flags, wide_count);
     hipLaunchKernelGGL(gqf_rebuild_wide_class, dim3(ncls * kWideClassGrid), dim3(64), 0, s, qf_dev, items, counts,
-                       region_start, parity, mode, wide_flags, wide_count, ncls);
+                       region_start, parity, mode, wide_flags, wide_count, ncls,
+                       g_wide_max_polls.load(std::memory_order_relaxed));
     gqf_prof_end("rebuild_wide", s);
   }
 }

@@ -45,7 +45,7 @@ EXPORTS = [
     "qf_server_request_times", "qf_server_get_stats", "qf_server_filter", "qf_server_destroy",
     "qf_malloc_device_kv", "qf_enumerate", "qf_get_stats", "qf_cluster_census", "qf_serialize",
     "qf_deserialize_device", "qf_resize_device", "qf_merge_device", "qf_multi_merge_device",
-    "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust",
+    "qf_bulk_mutate_seq", "qf_bulk_get_seq_adjust", "qf_set_wide_wait_bound",
 ]
 
 QF_SEQ_MAX_REQUESTS = 256
@@ -134,6 +134,8 @@ def load_library(path=LIB_PATH):
     L.gqf_group_set_workgroups_per_cu.restype = ctypes.c_int
     L.gqf_group_get_workgroups_per_cu.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
     L.gqf_group_get_workgroups_per_cu.restype = ctypes.c_int
+    L.qf_set_wide_wait_bound.argtypes = [ctypes.c_uint32]
+    L.qf_set_wide_wait_bound.restype = ctypes.c_uint32
     L.gqf_group_set_wait_bound.argtypes = [_vp, ctypes.c_uint32]
     L.gqf_group_set_wait_bound.restype = ctypes.c_int
     L.qf_check_layout.argtypes = [_vp, ctypes.POINTER(_u64)]
@@ -540,6 +542,14 @@ def autotune(nbits, load):
     if rc != 0:
         raise RuntimeError(f"gqf_autotune rc={rc}")
     return r.value, d.value
+
+
+def set_wide_wait_bound(max_polls):
+    """Bound on the polls of each wait between the wide pass's classes
+    (process-wide, default 2^24); returns the previous bound.  A wait that
+    runs out leaves its class's regions unchanged and sets the filter's
+    QF_STATUS_CLUSTER_OVERFLOW: a hook for tests of that path."""
+    return int(load_library().qf_set_wide_wait_bound(int(max_polls)))
 
 
 def fill_splitmix64(out, seed, skip=0, stream=None):

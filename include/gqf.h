@@ -125,6 +125,14 @@ int qf_reserve(QF *qf, uint64_t max_batch);
  * window is min(64, 2 x region) minus that), and radix digit bits of the
  * bulk sort, 6..10; the results do not depend on them, only the speed does */
 int qf_set_tuning(QF *qf, uint32_t region_blocks, uint32_t sort_digit_bits);
+/* No counterpart in the reference.  Bound on the polls of each wait between
+ * the classes of the wide pass (more than 64 regions of a rebuild phase
+ * whose push outran their window; default 2^24, which no correct run
+ * reaches).  A wait that runs out leaves that class's flagged regions
+ * unchanged and sets QF_STATUS_CLUSTER_OVERFLOW on the filter, so every
+ * kernel still exits.  Process-wide; returns the previous bound.  For
+ * tests of the failure path. */
+uint32_t qf_set_wide_wait_bound(uint32_t max_polls);
 /* stream-ordered variants of the bulk calls */
 void bulk_insert_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);
 void bulk_insert_reduce_async(QF *qf, uint64_t nvals, uint64_t *keys, uint8_t flags, void *stream);

@@ -249,6 +249,43 @@ def test_wide_window_fallback_at_high_load(gpu, oracle, nbits):
         o.close()
 
 
+def test_wide_class_wait_bound_zero_reports_overflow(gpu, oracle):
+    """The wide pass's classes wait for each other with a bounded poll.  At
+    bound 0 every class after the first gives up at once: its flagged
+    regions stay as they were, the filter reports CLUSTER_OVERFLOW, every
+    kernel exits, and the table's layout stays valid (the first class and
+    the phases still rebuilt theirs).  The default bound is restored
+    afterwards, and the same batches then give the oracle's image."""
+    from cqf_amd import gqf as G
+    from cqf_amd.gqf import to_device_u64
+    nbits = 22
+    _say(f"case wide class wait bound 0 q={nbits} R=56 load 0.95")
+    keys = oracle.splitmix64(77, int(0.95 * (1 << nbits)))
+    g = G.GQF(nbits)
+    prev = G.set_wide_wait_bound(0)
+    try:
+        g.set_tuning(56, 0)
+        for part in np.array_split(keys, 4):
+            g.bulk_insert(to_device_u64(part))
+        assert g.status() & G.QF_STATUS_CLUSTER_OVERFLOW
+        assert g.layout_errors() == 0
+    finally:
+        G.set_wide_wait_bound(prev)
+        g.close()
+    assert G.set_wide_wait_bound(prev) == prev  # restored
+    g, o = _gpu_and_oracle(oracle, nbits)
+    try:
+        g.set_tuning(56, 0)
+        for part in np.array_split(keys, 4):
+            g.bulk_insert(to_device_u64(part))
+            o.bulk_insert(part.copy())
+        _assert_images_equal(g.export_image(), o.image(), "insert after the bound is restored")
+        assert g.status() == 0 and g.layout_errors() == 0
+    finally:
+        g.close()
+        o.close()
+
+
 @pytest.mark.parametrize("nbits", [20, 22])
 def test_sparse_batches_after_dense_fill(gpu, oracle, nbits):
     """Batches of at most one key per point-lock region take the point path
