@@ -172,22 +172,54 @@ gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
 }
 
 // One launch over the keys of a filter group: key i belongs to the filter
-// whose segment [seg[f], seg[f+1]) holds i.
+// whose segment [seg[f], seg[f+1]) holds i (the last such f, so empty
+// segments are passed over).  Workgroup w takes the contiguous keys
+// [w chunk, (w + 1) chunk), chunk a multiple of 512, two per thread and
+// step (i and i + 256, both lines in flight as in bulk_get).  The filter
+// index only moves forward inside a chunk: one test of the next segment
+// start per key, a binary search only when it is passed.
+
+// the filter holding key i, given a filter lo with seg[lo] <= i
+__device__ inline uint32_t group_filter_from(const gqf_group_dev& g, uint64_t i, uint32_t lo) {
+  if (lo + 1 >= g.nf || g.seg[lo + 1] > i) return lo;
+  uint32_t hi = g.nf;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (g.seg[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 extern "C" __global__ void __launch_bounds__(256)
-gqf_group_get_kernel(gqf_group_dev g, const uint64_t* __restrict__ vals, uint64_t n, uint64_t* __restrict__ returns) {
+gqf_group_get_kernel(gqf_group_dev g, const uint64_t* __restrict__ vals, uint64_t n, uint64_t chunk,
+                     uint64_t* __restrict__ returns) {
   __shared__ uint32_t rows[256 * 17];
   uint32_t* row = rows + 17 * threadIdx.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t lo = 0, hi = g.nf;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (g.seg[mid] <= i) lo = mid; else hi = mid;
-    }
-    const QF& qf = g.descs[lo];
-    QF q;
-    q.blocks = qf.blocks;
-    q.nblocks = qf.nblocks;
-    returns[i] = gqf_count_hash(q, gqf_key_hash(qf, vals[i]), row);
+  const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+  if (c0 >= n) return;
+  const uint64_t c1 = n - c0 > chunk ? c0 + chunk : n;
+  uint32_t lo = group_filter_from(g, c0, 0);  // seg[0] = 0
+  for (uint64_t i = c0 + threadIdx.x; i < c1; i += 512) {
+    const uint64_t i1 = i + 256;
+    const bool two = i1 < c1;
+    const uint32_t f0 = group_filter_from(g, i, lo);
+    const uint32_t f1 = two ? group_filter_from(g, i1, f0) : f0;
+    const QF& d0 = g.descs[f0];
+    const QF& d1 = g.descs[f1];
+    QF q0, q1;
+    q0.blocks = d0.blocks;
+    q0.nblocks = d0.nblocks;
+    q1.blocks = d1.blocks;
+    q1.nblocks = d1.nblocks;
+    const uint64_t h0 = gqf_key_hash(d0, vals[i]);
+    const uint64_t h1 = two ? gqf_key_hash(d1, vals[i1]) : h0;
+    const HomeLine l0 = load_home(q0, h0);
+    const HomeLine l1 = load_home(q1, h1);
+    keep_line(l0);
+    keep_line(l1);
+    returns[i] = count_from_line(q0, h0, l0, row);
+    if (two) returns[i1] = count_from_line(q1, h1, l1, row);
+    lo = f1;
   }
 }
 
@@ -331,7 +363,10 @@ void gqf_launch_group_get(const gqf_group_dev& g, const uint64_t* vals, uint64_t
                           hipStream_t s) {
   if (n == 0) return;
   gqf_prof_begin("group_get", s);
-  hipLaunchKernelGGL(gqf_group_get_kernel, dim3(probe_grid(n)), dim3(256), 0, s, g, vals, n, returns);
+  // contiguous chunks of whole 512-key steps, one per workgroup
+  const uint32_t grid = probe_grid((n + 1) / 2);
+  const uint64_t chunk = ((n + grid - 1) / grid + 511) / 512 * 512;
+  hipLaunchKernelGGL(gqf_group_get_kernel, dim3(grid), dim3(256), 0, s, g, vals, n, chunk, returns);
   gqf_prof_end("group_get", s);
 }
 

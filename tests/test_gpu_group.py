@@ -152,6 +152,42 @@ def test_group_exact_counts_larger(gpu):
         _close(filters, group)
 
 
+def test_group_get_empty_and_short_segments(gpu):
+    """Group queries whose segments are empty (first, middle and last
+    filter), one key long, or end inside a workgroup's chunk of keys: every
+    key's count equals its own filter's bulk_get."""
+    from cqf_amd.gqf import GQF, GQFGroup
+    qs = [10, 12, 11, 13, 10, 12]
+    sizes = [0, 700, 0, 1, 5000, 0]
+    _say(f"case group get segments {sizes} over q {qs}")
+    gen = torch.Generator().manual_seed(11)
+    filters = [GQF(q) for q in qs]
+    group = GQFGroup(filters)
+    try:
+        pools = []
+        for f, q in zip(filters, qs):
+            ins = torch.randint(0, 2**62, (1 << (q - 1),), generator=gen, dtype=torch.int64)
+            f.bulk_insert(ins.clone().cuda())
+            pools.append(ins)
+        segs = []
+        for f, n in enumerate(sizes):
+            # half inserted keys (some twice), half fresh ones
+            pick = pools[f][torch.randint(0, pools[f].numel(), (n // 2,), generator=gen)]
+            fresh = torch.randint(0, 2**62, (n - n // 2,), generator=gen, dtype=torch.int64)
+            segs.append(torch.cat([pick, fresh]))
+        offsets = [0]
+        for sg in segs:
+            offsets.append(offsets[-1] + sg.numel())
+        keys = torch.cat(segs).cuda()
+        got = group.bulk_get(keys, offsets)
+        for f, sg in enumerate(segs):
+            if sg.numel():
+                want = filters[f].bulk_get(sg.cuda())
+                assert torch.equal(got[offsets[f]:offsets[f + 1]], want), f
+    finally:
+        _close(filters, group)
+
+
 def test_autotune_choices_keep_results(gpu, oracle):
     from cqf_amd import gqf
     _say("case autotune q=14 load 0.9")
