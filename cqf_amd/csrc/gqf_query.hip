@@ -13,8 +13,9 @@
 #include "gqf_internal.h"
 #include "gqf_launch.h"
 
-// Count of `hash` (< range) in the table.  `row` is the calling thread's
-// 17-word LDS scratch row.
+// Count of `hash` (< range) in the table (count_from_line, given the home
+// block's line from load_home).  `row` is the calling thread's 17-word LDS
+// scratch row.
 //
 // One round of loads fetches the home block's whole line (header and the
 // 64 slots, six 16-byte loads to one 128-byte line).  When the quotient's
@@ -42,8 +43,8 @@ __device__ __forceinline__ HomeLine load_home(const QF& qf, uint64_t hash) {
 
 // the line's registers taken as read here: every load of the line is issued
 // before the decode branches (the compiler would otherwise sink the slot
-// loads behind the header test, a second round), and in bulk_get both keys'
-// lines are in flight before either is decoded
+// loads behind the header test, a second round), and both keys' lines of a
+// step are in flight before either is decoded
 __device__ __forceinline__ void keep4(const uint4& v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
 __device__ __forceinline__ void keep_line(const HomeLine& l) {
   keep4(l.h0); keep4(l.h1); keep4(l.s0); keep4(l.s1); keep4(l.s2); keep4(l.s3);
@@ -125,12 +126,6 @@ __device__ uint64_t count_from_line(const QF& qf, uint64_t hash, const HomeLine&
   return 0;
 }
 
-__device__ uint64_t gqf_count_hash(const QF& qf, uint64_t hash, uint32_t* row) {
-  const HomeLine l = load_home(qf, hash);
-  keep_line(l);
-  return count_from_line(qf, hash, l, row);
-}
-
 extern "C" __global__ void __launch_bounds__(256)
 gqf_bulk_get_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ vals, uint64_t n,
                     uint64_t* __restrict__ returns) {
@@ -162,9 +157,19 @@ gqf_bulk_misses_kernel(const QF* __restrict__ qfp, const uint64_t* __restrict__ 
   const QF qf = *qfp;
   uint32_t* row = rows + 17 * threadIdx.x;
   uint32_t mine = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    mine += gqf_count_hash(qf, gqf_key_hash(qf, __builtin_nontemporal_load(&vals[i])), row) < 1;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // keys i and i + stride per step, both lines in flight (as bulk_get)
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    const uint64_t i1 = i + stride;
+    const bool two = i1 < n;
+    const uint64_t h0 = gqf_key_hash(qf, __builtin_nontemporal_load(&vals[i]));
+    const uint64_t h1 = two ? gqf_key_hash(qf, __builtin_nontemporal_load(&vals[i1])) : h0;
+    const HomeLine l0 = load_home(qf, h0);
+    const HomeLine l1 = load_home(qf, h1);
+    keep_line(l0);
+    keep_line(l1);
+    mine += count_from_line(qf, h0, l0, row) < 1;
+    if (two) mine += count_from_line(qf, h1, l1, row) < 1;
   }
   // wave reduction, one atomic per wave
   for (int d = 32; d >= 1; d >>= 1) mine += __shfl_down(mine, d, 64);
@@ -245,7 +250,7 @@ void gqf_launch_bulk_get(const QF* qf_dev, const uint64_t* vals, uint64_t n, uin
 void gqf_launch_misses(const QF* qf_dev, const uint64_t* vals, uint64_t n, unsigned long long* misses,
                        hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(gqf_bulk_misses_kernel, dim3(probe_grid(n)), dim3(256), 0, s, qf_dev, vals, n, misses);
+  hipLaunchKernelGGL(gqf_bulk_misses_kernel, dim3(probe_grid((n + 1) / 2)), dim3(256), 0, s, qf_dev, vals, n, misses);
 }
 
 // ---------------------------------------------------------------------------
