@@ -249,6 +249,43 @@ __device__ __forceinline__ void lean_write(Smem<W>& sm, uint32_t lo, uint32_t hi
   if (used > 8) atomicOr(&wp[pad <= 2 ? 3 : 2], y2);
 }
 
+// The straight-line delete merge: one or two items, each removing one
+// instance (an absent key's removal is a no-op), on an old run of at most 8
+// strictly ascending bytes.  Such a run holds count-1 counters only (a
+// count of 2 or more repeats its remainder), so each item's remainder, if
+// present, loses its one byte and the rest stay in order.  Returns false
+// (the general merge, merge_quotient) otherwise; lo / hi / len as
+// lean_read's.  Reads only the old rows, so the measuring and the writing
+// pass can each call it.
+template <int W>
+__device__ __forceinline__ bool lean_delete(const Smem<W>& sm, const Items& src, uint32_t it0, uint32_t it1,
+                                            int32_t st_o, uint32_t old_len, uint32_t h_first, uint32_t& lo,
+                                            uint32_t& hi, uint32_t& len) {
+  const uint32_t n = it1 - it0;
+  if (old_len == 0 || old_len > 8 || n > 2) return false;
+  const uint32_t r0 = h_first & 0xFFu;
+  const uint32_t r1 = n == 2 ? (src.hash(sm.items, it0 + 1) & 0xFFu) : r0;
+  uint32_t x0, x1;
+  read_slots8(sm, st_o, x0, x1);
+  bool asc = true;
+  uint32_t prev = 0, kept = 0;
+  uint64_t y = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t b = ((k < 4 ? x0 : x1) >> (8 * (k & 3))) & 0xFFu;
+    const bool in = (uint32_t)k < old_len;
+    if (k > 0 && in && b <= prev) asc = false;
+    prev = b;
+    const bool keep = in && b != r0 && b != r1;
+    y |= keep ? (uint64_t)b << (8u * kept) : 0ULL;
+    kept += keep ? 1u : 0u;
+  }
+  lo = (uint32_t)y;
+  hi = (uint32_t)(y >> 32);
+  len = kept;
+  return asc;
+}
+
 // ---------------------------------------------------------------------------
 // Two-pointer merge of the old counters of one quotient (ascending
 // remainders, old run at st_o .. st_o + old_len - 1 of sin) with its items
@@ -1303,8 +1340,14 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
       uint32_t old_len;
       uint32_t nl = 0;
       if (locate_old_run(sm, nwin, q, st_o, old_len)) {
+        const uint32_t it0 = sm.tit0[t];
         const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-        nl = merge_quotient<W, false>(sm, src, sm.tit0[t], it1, st_o, old_len, 0, src.hash(sm.items, sm.tit0[t]));
+        const uint32_t h0 = src.hash(sm.items, it0);
+        uint32_t lo, hi, ln;
+        if (kStash && kind == kSub && lean_delete(sm, src, it0, it1, st_o, old_len, h0, lo, hi, ln))
+          nl = ln;
+        else
+          nl = merge_quotient<W, false>(sm, src, it0, it1, st_o, old_len, 0, h0);
       } else {
         atomicMin(&sm.first_invalid, q >> 6);
         st_o = q;
@@ -1434,9 +1477,16 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
     const uint32_t nq = sm.nq;
     for (uint32_t t = lane; t < nq; t += 64) {
       if (sm.tnl[t] == 0) continue;
+      const uint32_t it0 = sm.tit0[t];
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      merge_quotient<W, true>(sm, src, sm.tit0[t], it1, sm.tst[t], sm.tol[t], sm.tsn[t],
-                              src.hash(sm.items, sm.tit0[t]));
+      const uint32_t h0 = src.hash(sm.items, it0);
+      uint32_t lo, hi, ln;
+      // the same straight-line delete the measuring pass took (not in a
+      // caller short of registers, kStash false)
+      if (kStash && kind == kSub && lean_delete(sm, src, it0, it1, sm.tst[t], sm.tol[t], h0, lo, hi, ln))
+        lean_write(sm, lo, hi, ln, sm.tsn[t]);
+      else
+        merge_quotient<W, true>(sm, src, it0, it1, sm.tst[t], sm.tol[t], sm.tsn[t], h0);
     }
   }
   __syncthreads();
