@@ -592,6 +592,38 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
     w = sm.run_in[bj] & (~0ULL << ((xo + 1) & 63));
   }
   uint64_t u = occ | touched;
+  if constexpr (!WRITE) {
+    // Pass 1 without branches apart from the step to the next run-end
+    // word: each quotient's old run end, its table length when listed, and
+    // the fold of its run into the aggregate are computed on every lane and
+    // selected, so the lanes stay converged through the loop.
+    while (u) {
+      const int q = __builtin_ctzll(u);
+      const uint64_t bit = 1ULL << q;
+      u ^= bit;
+      const int32_t qa = blk_base + q;
+      const bool occupied = (occ & bit) != 0;
+      if (occupied && !w) {
+        do {
+          if (++bj >= nwin) return false;
+          w = sm.run_in[bj];
+        } while (!w);
+      }
+      const int32_t en_o = 64 * bj + __builtin_ctzll(w | (1ULL << 63));  // w != 0 when occupied
+      const int32_t st_o = qa > xo + 1 ? qa : xo + 1;
+      uint32_t t = tbase + (uint32_t)__popcll(touched & (bit - 1));
+      t = t < (uint32_t)kItemCap ? t : (uint32_t)kItemCap - 1u;  // read on every lane, used when listed
+      const uint32_t tl = sm.tnl[t];
+      const uint32_t len = (touched & bit) ? tl : (occupied ? (uint32_t)(en_o - st_o + 1) : 0u);
+      w = occupied ? (w & (w - 1)) : w;
+      xo = occupied ? en_o : xo;
+      const int32_t na = agg_a + (int32_t)len, la = qa + (int32_t)len - 1;
+      agg_a = len ? (na > la ? na : la) : agg_a;  // compose(agg, (qa + len - 1, len)) for a run
+      agg_b += (int32_t)len;
+      newocc |= len ? bit : 0ULL;
+    }
+    return true;
+  }
   while (u) {
     const int q = __builtin_ctzll(u);
     const uint64_t bit = 1ULL << q;
