@@ -624,54 +624,57 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
     }
     return true;
   }
+  // Pass 2: the runs written at their new starts.  Only the listed
+  // quotients' table update is a branch; an unlisted run's slots are copied
+  // on every lane (a listed one copies 0 bytes: its counters are written
+  // after the walk) and the bit updates are selects.
   while (u) {
     const int q = __builtin_ctzll(u);
     const uint64_t bit = 1ULL << q;
     u ^= bit;
     const int32_t qa = blk_base + q;
-    int32_t st_o = 0, en_o = -1;
-    if (occ & bit) {
-      while (!w) {
+    const bool occupied = (occ & bit) != 0;
+    if (occupied && !w) {
+      do {
         if (++bj >= nwin) return false;
         w = sm.run_in[bj];
-      }
-      en_o = 64 * bj + __builtin_ctzll(w);
-      w &= w - 1;
-      st_o = qa > xo + 1 ? qa : xo + 1;
-      xo = en_o;
+      } while (!w);
     }
-    int32_t st_n = 0;
-    if (WRITE) st_n = qa > xn + 1 ? qa : xn + 1;
-    uint32_t len;
+    const int32_t en_o = 64 * bj + __builtin_ctzll(w | (1ULL << 63));  // w != 0 when occupied
+    const int32_t st_o = qa > xo + 1 ? qa : xo + 1;
+    w = occupied ? (w & (w - 1)) : w;
+    xo = occupied ? en_o : xo;
+    const int32_t st_n = qa > xn + 1 ? qa : xn + 1;
+    uint32_t len, copy;
     if (touched & bit) {
       // lengths from the table; the counters are written after the walk
       const uint32_t t = tbase + (uint32_t)__popcll(touched & (bit - 1));
       len = sm.tnl[t];
-      if (WRITE) sm.tsn[t] = (int16_t)st_n;
+      sm.tsn[t] = (int16_t)st_n;
+      copy = 0;
     } else {
-      len = (uint32_t)(en_o - st_o + 1);
-      if (WRITE) {
-        if (len <= 8) {
-          // as words: the old slots read aligned, ORed into the cleared rows
-          uint32_t x0, x1;
-          read_slots8(sm, st_o, x0, x1);
-          const uint64_t x = ((uint64_t)x1 << 32 | x0) & gqf_mask(8u * len);
-          lean_write(sm, (uint32_t)x, (uint32_t)(x >> 32), len, st_n);
-        } else {
-          for (uint32_t k = 0; k < len; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
-        }
-      }
+      len = copy = (uint32_t)(en_o - st_o + 1);  // u = occ | touched: an unlisted quotient is occupied
     }
-    if (len) {
-      newocc |= bit;
-      compose(agg_a, agg_b, qa + (int32_t)len - 1, (int32_t)len);
-      if (WRITE) {
-        const int32_t en_n = st_n + (int32_t)len - 1;
-        atomicOr((unsigned long long*)&sm.run_out[en_n >> 6], 1ULL << (en_n & 63));
-        rb.add(st_n - blk_base, en_n - blk_base);
-        xn = en_n;
-      }
+    if (copy <= 8) {
+      // as words: the old slots read aligned, ORed into the cleared rows (a
+      // listed run ORs 0 into its block's first word)
+      uint32_t x0, x1;
+      read_slots8(sm, copy ? st_o : blk_base, x0, x1);
+      const uint64_t x = ((uint64_t)x1 << 32 | x0) & gqf_mask(8u * copy);
+      lean_write(sm, (uint32_t)x, (uint32_t)(x >> 32), copy, copy ? st_n : blk_base);
+    } else {
+      for (uint32_t k = 0; k < copy; k++) out[sin_off(st_n + (int32_t)k)] = in[sin_off(st_o + (int32_t)k)];
     }
+    // the run's end bit, its start and end bits inside the block (RunBits),
+    // the new end-before; nothing for a run deleted to length 0
+    const int32_t en_n = st_n + (int32_t)len - 1;
+    const int32_t ei = len ? en_n : blk_base;
+    atomicOr((unsigned long long*)&sm.run_out[ei >> 6], len ? 1ULL << (ei & 63) : 0ULL);
+    const int32_t rs = st_n - blk_base, re = en_n - blk_base;  // rs >= 0: st_n > xnew[j] >= 64 j - 1
+    const bool in_blk = len && rs < 64;
+    rb.s |= in_blk ? 1ULL << (rs & 63) : 0ULL;
+    rb.e |= in_blk ? 1ULL << (re < 63 ? re : 63) : 0ULL;
+    xn = len ? en_n : xn;
   }
   return true;
 }
