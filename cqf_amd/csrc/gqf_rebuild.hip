@@ -1366,10 +1366,16 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
     if (shift_ok && fast && kind == kAdd)
       return rebuild_insert_shift<W, kStash>(qf, B0, R, nwin, src, m, sm, occ_w[0], used_w[0], tm_w[0], table_end_in_window);
   }
+  // With kStash, lane takes listed quotients lane + 64 k in round k (at
+  // most kItemCap = 4 x 64 of them), and a delete that takes the
+  // straight-line merge keeps its bytes in registers (del_lo / del_hi, bit k
+  // of del_lean) for the writing pass; without it (a caller short of
+  // registers) both passes call merge_quotient.
+  uint32_t del_lo[4], del_hi[4], del_lean = 0;
   if (fast) {
     // old run and new length of every listed quotient, one quotient per lane
     const uint32_t nq = sm.nq;
-    for (uint32_t t = lane; t < nq; t += 64) {
+    auto measure = [&](uint32_t t, int k) {
       const int32_t q = (int32_t)(sm.items[sm.tit0[t]] >> 8);
       int32_t st_o;
       uint32_t old_len;
@@ -1378,11 +1384,13 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
         const uint32_t it0 = sm.tit0[t];
         const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
         const uint32_t h0 = src.hash(sm.items, it0);
-        uint32_t lo, hi, ln;
-        if (kStash && kind == kSub && lean_delete(sm, src, it0, it1, st_o, old_len, h0, lo, hi, ln))
+        uint32_t ln;
+        if (kStash && kind == kSub && lean_delete(sm, src, it0, it1, st_o, old_len, h0, del_lo[k], del_hi[k], ln)) {
           nl = ln;
-        else
+          del_lean |= 1u << k;
+        } else {
           nl = merge_quotient<W, false>(sm, src, it0, it1, st_o, old_len, 0, h0);
+        }
       } else {
         atomicMin(&sm.first_invalid, q >> 6);
         st_o = q;
@@ -1395,6 +1403,14 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
       sm.tst[t] = (int16_t)st_o;
       sm.tol[t] = (RunLen<W>)old_len;
       sm.tnl[t] = (RunLen<W>)nl;
+    };
+    if constexpr (kStash) {
+      static_assert(kItemCap <= 4 * 64, "four rounds of listed quotients");
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((uint32_t)lane + 64u * (uint32_t)k < nq) measure((uint32_t)lane + 64u * (uint32_t)k, k);
+    } else {
+      for (uint32_t t = lane; t < nq; t += 64) measure(t, 0);
     }
   }
   __syncthreads();
@@ -1510,18 +1526,22 @@ __device__ __forceinline__ int rebuild_region_lines(const QF& qf, uint64_t r, co
   if (fast) {
     // counters of the listed quotients at their new starts
     const uint32_t nq = sm.nq;
-    for (uint32_t t = lane; t < nq; t += 64) {
-      if (sm.tnl[t] == 0) continue;
+    auto write = [&](uint32_t t, int k) {
+      if (sm.tnl[t] == 0) return;
+      if (kStash && ((del_lean >> k) & 1u)) {  // the measuring pass's straight-line delete
+        lean_write(sm, del_lo[k], del_hi[k], sm.tnl[t], sm.tsn[t]);
+        return;
+      }
       const uint32_t it0 = sm.tit0[t];
       const uint32_t it1 = t + 1 < nq ? sm.tit0[t + 1] : m;
-      const uint32_t h0 = src.hash(sm.items, it0);
-      uint32_t lo, hi, ln;
-      // the same straight-line delete the measuring pass took (not in a
-      // caller short of registers, kStash false)
-      if (kStash && kind == kSub && lean_delete(sm, src, it0, it1, sm.tst[t], sm.tol[t], h0, lo, hi, ln))
-        lean_write(sm, lo, hi, ln, sm.tsn[t]);
-      else
-        merge_quotient<W, true>(sm, src, it0, it1, sm.tst[t], sm.tol[t], sm.tsn[t], h0);
+      merge_quotient<W, true>(sm, src, it0, it1, sm.tst[t], sm.tol[t], sm.tsn[t], src.hash(sm.items, it0));
+    };
+    if constexpr (kStash) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((uint32_t)lane + 64u * (uint32_t)k < nq) write((uint32_t)lane + 64u * (uint32_t)k, k);
+    } else {
+      for (uint32_t t = lane; t < nq; t += 64) write(t, 0);
     }
   }
   __syncthreads();
