@@ -592,6 +592,9 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
     w = sm.run_in[bj] & (~0ULL << ((xo + 1) & 63));
   }
   uint64_t u = occ | touched;
+  // the table entry of the block's next listed quotient: they are visited in
+  // ascending order, so the k-th one met is entry tbase + k
+  uint32_t tn = tbase;
   if constexpr (!WRITE) {
     // Pass 1 without branches apart from the step to the next run-end
     // word: each quotient's old run end, its table length when listed, and
@@ -611,10 +614,10 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
       }
       const int32_t en_o = 64 * bj + __builtin_ctzll(w | (1ULL << 63));  // w != 0 when occupied
       const int32_t st_o = qa > xo + 1 ? qa : xo + 1;
-      uint32_t t = tbase + (uint32_t)__popcll(touched & (bit - 1));
-      t = t < (uint32_t)kItemCap ? t : (uint32_t)kItemCap - 1u;  // read on every lane, used when listed
-      const uint32_t tl = sm.tnl[t];
-      const uint32_t len = (touched & bit) ? tl : (occupied ? (uint32_t)(en_o - st_o + 1) : 0u);
+      const uint32_t tl = sm.tnl[tn < (uint32_t)kItemCap ? tn : (uint32_t)kItemCap - 1u];  // used when listed
+      const bool listed = (touched & bit) != 0;
+      const uint32_t len = listed ? tl : (occupied ? (uint32_t)(en_o - st_o + 1) : 0u);
+      tn += listed ? 1u : 0u;
       w = occupied ? (w & (w - 1)) : w;
       xo = occupied ? en_o : xo;
       const int32_t na = agg_a + (int32_t)len, la = qa + (int32_t)len - 1;
@@ -648,9 +651,9 @@ __device__ __forceinline__ bool walk_block_table(Smem<W>& sm, int j, int nwin, u
     uint32_t len, copy;
     if (touched & bit) {
       // lengths from the table; the counters are written after the walk
-      const uint32_t t = tbase + (uint32_t)__popcll(touched & (bit - 1));
-      len = sm.tnl[t];
-      sm.tsn[t] = (int16_t)st_n;
+      len = sm.tnl[tn];
+      sm.tsn[tn] = (int16_t)st_n;
+      tn++;
       copy = 0;
     } else {
       len = copy = (uint32_t)(en_o - st_o + 1);  // u = occ | touched: an unlisted quotient is occupied
